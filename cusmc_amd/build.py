@@ -1,0 +1,95 @@
+"""Build the native parts in-tree: libcusmc.so (front end + gfx950 kernels +
+C ABI), the `cusmc` command-line tool, and the test oracle.
+
+Plain hipcc / g++ invocations, objects compiled in parallel; no CMake.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+PKG = ROOT / "cusmc_amd"
+CSRC = PKG / "csrc"
+BUILD = PKG / "_build"
+LIB = PKG / "lib"
+ORACLE = ROOT / "oracle"
+
+ARCH = os.environ.get("SMC_OFFLOAD_ARCH", "gfx950")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+CXX = shutil.which("g++") or "g++"
+
+COMMON = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+HOST_SOURCES = ["network.cpp", "uppaal.cpp", "compile.cpp"]
+HIP_SOURCES = ["capi.cpp", "sim_kernel.hip"]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and "not a recognized feature" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+
+
+def _obj(src: str) -> Path:
+    return BUILD / (src.replace(".", "_") + ".o")
+
+
+def build_lib() -> Path:
+    BUILD.mkdir(parents=True, exist_ok=True)
+    LIB.mkdir(parents=True, exist_ok=True)
+    jobs = []
+    for s in HOST_SOURCES:
+        jobs.append([CXX, *COMMON, "-Wall", "-c", str(CSRC / s), "-o", str(_obj(s))])
+    for s in HIP_SOURCES:
+        extra = ["-x", "hip"] if s.endswith(".hip") else []
+        jobs.append([HIPCC, *COMMON, *HIP_FLAGS, *extra, "-c", str(CSRC / s), "-o", str(_obj(s))])
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        list(ex.map(_run, jobs))
+    out = LIB / "libcusmc.so"
+    tmp = LIB / "libcusmc.so.tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
+          *[str(_obj(s)) for s in HOST_SOURCES + HIP_SOURCES]])
+    os.replace(tmp, out)
+    return out
+
+
+def build_cli() -> Path:
+    out = LIB / "cusmc"
+    tmp = LIB / "cusmc.tmp"
+    _run([CXX, *COMMON, "-Wall", str(CSRC / "cli_main.cpp"), "-o", str(tmp),
+          f"-L{LIB}", "-lcusmc", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle() -> Path:
+    out = ORACLE / "liboracle.so"
+    tmp = ORACLE / "liboracle.so.tmp"
+    _run([CXX, *COMMON, "-Wall", "-shared", "-pthread", str(ORACLE / "oracle.cpp"), "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> None:
+    with cf.ThreadPoolExecutor(max_workers=2) as ex:
+        f_lib = ex.submit(build_lib)
+        f_or = ex.submit(build_oracle)
+        f_lib.result()
+        f_or.result()
+    if (CSRC / "cli_main.cpp").exists():
+        build_cli()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "lib": build_lib, "cli": build_cli, "oracle": build_oracle}[w]()
+    print("built", ", ".join(what))

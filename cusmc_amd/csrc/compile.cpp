@@ -1,0 +1,248 @@
+// compile.cpp — lowers an smc_network to the run kernel's tables.
+//
+// What the reference does at this point (shared_model_state.cu:164-689,
+// pn_compile_visitor.cpp) and what is done here instead:
+//   * expressions: cuSMC keeps guard right-hand sides as pointer trees and
+//     compiles the rest to a Polish-notation array evaluated with a
+//     10-double stack in local memory (expressions.cu:97-119).  Here every
+//     expression becomes an inline constant, an inline variable or a short
+//     three-address program over four registers (Sethi-Ullman order), with
+//     identical programs and constants shared across components.
+//   * invariants: only conjuncts that can bound a delay are kept — a clock
+//     on the left, `<` or `<=` (simulation.cu:464-509); the kernel never
+//     reads any other invariant.
+//   * guards: only conjuncts on a variable are kept (simulation.cu:209); an
+//     edge with a `!=` conjunct can never be enabled (simulation.cu:249-251)
+//     and is flagged instead.
+//   * locations get a delay class so the kernel does not re-derive it.
+#include "compile.hpp"
+
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include "expr_ops.h"
+#include "network.hpp"
+
+namespace smc {
+
+namespace {
+
+class Lowerer {
+public:
+    Lowerer(const smc_network &n, ModelImage &img) : n_(n), img_(img) {}
+
+    uint32_t ref(int e, double dflt) {
+        if (e < 0) return SMC_REF_MAKE(SMC_REF_CONST, const_index(dflt));
+        const smc_expr_node &x = n_.exprs[e];
+        if (x.op == SMC_E_LIT) return SMC_REF_MAKE(SMC_REF_CONST, const_index(x.value));
+        if (x.op == SMC_E_RANDOM) return SMC_REF_MAKE(SMC_REF_CONST, const_index(1.0));
+        if (x.op == SMC_E_VAR) return SMC_REF_MAKE(SMC_REF_VAR, (uint32_t)x.var);
+        std::vector<DevInsn> prog;
+        int maxreg = 0;
+        uint16_t r = emit(e, 0, prog, &maxreg);
+        if (r != SMC_OPND(SMC_OPND_REG, 0)) throw std::runtime_error("internal: expression result not in r0");
+        if ((int)prog.size() > SMC_MAX_CODE_LEN)
+            throw std::runtime_error("limit: expression needs more than 63 instructions");
+        std::string key((const char *)prog.data(), prog.size() * sizeof(DevInsn));
+        auto it = code_cache_.find(key);
+        uint32_t first;
+        if (it != code_cache_.end()) first = it->second;
+        else {
+            first = (uint32_t)img_.code.size();
+            if (first + prog.size() >= (1u << 24)) throw std::runtime_error("limit: bytecode larger than 16M instructions");
+            img_.code.insert(img_.code.end(), prog.begin(), prog.end());
+            code_cache_[key] = first;
+        }
+        return SMC_REF_CODE_MAKE(first, prog.size());
+    }
+
+private:
+    const smc_network &n_;
+    ModelImage &img_;
+    std::unordered_map<uint64_t, uint32_t> const_cache_;
+    std::unordered_map<std::string, uint32_t> code_cache_;
+
+    uint32_t const_index(double v) {
+        uint64_t b;
+        memcpy(&b, &v, 8);
+        auto it = const_cache_.find(b);
+        if (it != const_cache_.end()) return it->second;
+        uint32_t i = (uint32_t)img_.consts.size();
+        if (i > SMC_MAX_OPND_INDEX) throw std::runtime_error("limit: more than 16384 distinct constants");
+        img_.consts.push_back(v);
+        const_cache_[b] = i;
+        return i;
+    }
+
+    int need(int e) const {
+        const smc_expr_node &x = n_.exprs[e];
+        if (x.op == SMC_E_LIT || x.op == SMC_E_VAR || x.op == SMC_E_RANDOM) return 0;
+        int ar = smc_op_arity(x.op);
+        const int kids[3] = {x.a, x.b, x.c};
+        int nk[3] = {0, 0, 0};
+        for (int i = 0; i < ar; ++i) nk[i] = need(kids[i]);
+        // evaluate neediest first; each evaluated non-leaf child holds one register
+        int order[3] = {0, 1, 2};
+        for (int i = 0; i < ar; ++i)
+            for (int j = i + 1; j < ar; ++j)
+                if (nk[order[j]] > nk[order[i]]) std::swap(order[i], order[j]);
+        int held = 0, peak = 1;
+        for (int i = 0; i < ar; ++i) {
+            int k = order[i];
+            if (nk[k] > 0) { peak = std::max(peak, held + nk[k]); ++held; }
+        }
+        return std::max(peak, 1);
+    }
+
+    uint16_t emit(int e, int base, std::vector<DevInsn> &prog, int *maxreg) {
+        const smc_expr_node &x = n_.exprs[e];
+        if (x.op == SMC_E_LIT) return SMC_OPND(SMC_OPND_CONST, const_index(x.value));
+        if (x.op == SMC_E_RANDOM) return SMC_OPND(SMC_OPND_CONST, const_index(1.0));
+        if (x.op == SMC_E_VAR) {
+            if (x.var > SMC_MAX_OPND_INDEX) throw std::runtime_error("limit: variable index above 16383 in an expression");
+            return SMC_OPND(SMC_OPND_VAR, (uint32_t)x.var);
+        }
+        int ar = smc_op_arity(x.op);
+        const int kids[3] = {x.a, x.b, x.c};
+        int nk[3] = {0, 0, 0};
+        for (int i = 0; i < ar; ++i) nk[i] = need(kids[i]);
+        int order[3] = {0, 1, 2};
+        for (int i = 0; i < ar; ++i)
+            for (int j = i + 1; j < ar; ++j)
+                if (nk[order[j]] > nk[order[i]]) std::swap(order[i], order[j]);
+        uint16_t opnd[3] = {0, 0, 0};
+        int r = base;
+        for (int i = 0; i < ar; ++i) {
+            int k = order[i];
+            opnd[k] = emit(kids[k], r, prog, maxreg);
+            if ((opnd[k] >> 14) == SMC_OPND_REG) ++r;
+        }
+        if (base >= SMC_NREGS) throw std::runtime_error("limit: expression needs more than 4 registers");
+        *maxreg = std::max(*maxreg, base);
+        DevInsn in{};
+        in.op = (uint8_t)x.op;
+        in.dst = (uint8_t)base;
+        in.a = opnd[0]; in.b = opnd[1]; in.c = opnd[2];
+        prog.push_back(in);
+        return SMC_OPND(SMC_OPND_REG, (uint32_t)base);
+    }
+};
+
+}  // namespace
+
+ModelImage compile_model(const smc_network &n) {
+    std::string err = validate(n);
+    if (!err.empty()) throw std::runtime_error(err);
+    ModelImage img;
+    img.n_comp = n.n_components;
+    img.n_vars = n.n_vars;
+    int leaves = 1;
+    while (leaves < n.n_components) leaves <<= 1;
+    img.n_leaves = leaves;
+    Lowerer lw(n, img);
+
+    img.var_init.assign(n.var_init, n.var_init + n.n_vars);
+    img.var_kind.assign(n.var_kind, n.var_kind + n.n_vars);
+    for (int v = 0; v < n.n_vars; ++v)
+        if (n.var_kind[v] == SMC_VAR_CLOCK) {
+            if (v > 65535) throw std::runtime_error("limit: clock variable index above 65535");
+            img.clocks.push_back((uint16_t)v);
+        }
+    img.comp_init_loc.assign(n.comp_init_loc, n.comp_init_loc + n.n_components);
+
+    for (int l = 0; l < n.n_locations; ++l) {
+        const smc_location &L = n.locations[l];
+        DevLoc d{};
+        d.type = (uint8_t)L.type;
+        d.rate = lw.ref(L.rate, 1.0);
+        d.inv_first = (uint32_t)img.invs.size();
+        for (int k = 0; k < L.inv_count; ++k) {
+            const smc_constraint &C = n.constraints[L.inv_first + k];
+            if (!C.uses_variable) continue;
+            if (n.var_kind[C.var] != SMC_VAR_CLOCK) continue;
+            if (C.op != SMC_C_LE && C.op != SMC_C_LT) continue;
+            DevInv iv{};
+            iv.var = (uint16_t)C.var;
+            iv.strict = C.op == SMC_C_LT;
+            iv.bound = lw.ref(C.rhs, 0.0);
+            img.invs.push_back(iv);
+        }
+        size_t ninv = img.invs.size() - d.inv_first;
+        if (ninv > 65535) throw std::runtime_error("limit: more than 65535 invariants on a location");
+        d.n_inv = (uint16_t)ninv;
+        d.klass = L.type > SMC_LOC_GOAL ? SMC_K_ZERO : (ninv > 0 ? SMC_K_BOUNDED : SMC_K_EXP);
+        if (L.edge_count > SMC_MAX_EDGES_PER_LOC)
+            throw std::runtime_error("limit: a location has more than 64 outgoing edges");
+        d.edge_first = (uint32_t)L.edge_first;
+        img.locs.push_back(d);
+        img.loc_edge_n.push_back((uint8_t)L.edge_count);
+    }
+    for (int e = 0; e < n.n_edges; ++e) {
+        const smc_edge &E = n.edges[e];
+        DevEdge d{};
+        d.dst = E.dst;
+        d.channel = E.channel;
+        d.weight = lw.ref(E.weight, 1.0);
+        d.dst_goal = n.locations[E.dst].type == SMC_LOC_GOAL;
+        d.guard_first = (uint32_t)img.guards.size();
+        for (int k = 0; k < E.guard_count; ++k) {
+            const smc_constraint &C = n.constraints[E.guard_first + k];
+            if (!C.uses_variable) continue;
+            if (C.op == SMC_C_NE) { d.never = 1; continue; }
+            if (C.var > 65535) throw std::runtime_error("limit: guarded variable index above 65535");
+            DevGuard g{};
+            g.var = (uint16_t)C.var;
+            g.op = (uint8_t)C.op;
+            g.rhs = lw.ref(C.rhs, 0.0);
+            img.guards.push_back(g);
+        }
+        size_t ng = img.guards.size() - d.guard_first;
+        if (ng > 65535) throw std::runtime_error("limit: more than 65535 guards on an edge");
+        d.guard_n = (uint16_t)ng;
+        d.upd_first = (uint32_t)img.upds.size();
+        for (int k = 0; k < E.upd_count; ++k) {
+            const smc_update &U = n.updates[E.upd_first + k];
+            if (U.var > 65535) throw std::runtime_error("limit: assigned variable index above 65535");
+            DevUpd u{};
+            u.var = (uint16_t)U.var;
+            u.expr = lw.ref(U.expr, 0.0);
+            img.upds.push_back(u);
+        }
+        if (E.upd_count > 65535) throw std::runtime_error("limit: more than 65535 updates on an edge");
+        d.upd_n = (uint16_t)E.upd_count;
+        img.edges.push_back(d);
+    }
+    // keep every table non-empty so device pointers are valid
+    if (img.invs.empty()) img.invs.push_back(DevInv{});
+    if (img.edges.empty()) img.edges.push_back(DevEdge{});
+    if (img.guards.empty()) img.guards.push_back(DevGuard{});
+    if (img.upds.empty()) img.upds.push_back(DevUpd{});
+    if (img.consts.empty()) img.consts.push_back(0.0);
+    if (img.code.empty()) img.code.push_back(DevInsn{});
+    if (img.var_init.empty()) img.var_init.push_back(0.0);
+    if (img.clocks.empty()) img.clocks.push_back(0);
+    return img;
+}
+
+double eval_ref_host(const ModelImage &m, uint32_t ref, const double *vars) {
+    uint32_t kind = SMC_REF_KIND(ref);
+    if (kind == SMC_REF_CONST) return m.consts[ref & 0x3FFFFFFFu];
+    if (kind == SMC_REF_VAR) return vars[ref & 0x3FFFFFFFu];
+    uint32_t first = ref & 0xFFFFFFu, count = (ref >> 24) & 0x3Fu;
+    double reg[SMC_NREGS] = {0, 0, 0, 0};
+    auto fetch = [&](uint16_t o) -> double {
+        uint32_t k = o >> 14, i = o & 0x3FFFu;
+        return k == SMC_OPND_REG ? reg[i & 3] : k == SMC_OPND_VAR ? vars[i] : m.consts[i];
+    };
+    for (uint32_t k = 0; k < count; ++k) {
+        const DevInsn &in = m.code[first + k];
+        int ar = smc_op_arity(in.op);
+        double a = ar > 0 ? fetch(in.a) : 0.0, b = ar > 1 ? fetch(in.b) : 0.0, c = ar > 2 ? fetch(in.c) : 0.0;
+        reg[in.dst & 3] = smc_apply_op(in.op, a, b, c);
+    }
+    return reg[0];
+}
+
+}  // namespace smc

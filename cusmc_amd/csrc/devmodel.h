@@ -1,0 +1,115 @@
+// devmodel.h — device-side model tables of the run kernel (host and device).
+//
+// cuSMC keeps a level-major NodeInfo table with 40-byte nodes, 152-byte
+// GuardInfo records and one device allocation per expression node
+// (simulation/state/shared_model_state.cuh:45-217, shared_model_state.cu:
+// 15-161).  Here the model is one arena of small fixed-size records indexed
+// by 32-bit offsets, and every expression is either an inline constant, an
+// inline variable reference or a short run of register bytecode:
+//
+//   expression reference (uint32):  bits 31..30 kind
+//       SMC_REF_CONST  index into consts[]
+//       SMC_REF_VAR    variable id
+//       SMC_REF_CODE   bits 0..23 first instruction, bits 24..29 count
+//
+//   instruction (8 bytes): op, dst register, up to three operands; an
+//   operand (uint16) is a register r0..r3, a variable or a constant:
+//       bits 15..14 kind (0 reg, 1 var, 2 const), bits 13..0 index.
+//   The value of an expression is register r0 after its last instruction.
+#pragma once
+
+#include <stdint.h>
+
+#define SMC_REF_CONST 0u
+#define SMC_REF_VAR 1u
+#define SMC_REF_CODE 2u
+#define SMC_REF_KIND(r) ((r) >> 30)
+#define SMC_REF_MAKE(kind, payload) (((uint32_t)(kind) << 30) | (uint32_t)(payload))
+#define SMC_REF_CODE_MAKE(first, count) (((uint32_t)SMC_REF_CODE << 30) | ((uint32_t)(count) << 24) | (uint32_t)(first))
+
+#define SMC_OPND_REG 0u
+#define SMC_OPND_VAR 1u
+#define SMC_OPND_CONST 2u
+#define SMC_OPND(kind, idx) ((uint16_t)(((kind) << 14) | (idx)))
+
+#define SMC_NREGS 4
+#define SMC_MAX_CODE_LEN 63
+#define SMC_MAX_OPND_INDEX 16383
+#define SMC_MAX_EDGES_PER_LOC 64
+
+// Delay classes of a location (simulation.cu:437-540).
+enum smc_delay_class {
+    SMC_K_ZERO = 0,     // branchpoint, urgent, committed: delay 0
+    SMC_K_EXP = 1,      // exponential at the location's rate
+    SMC_K_BOUNDED = 2   // uniform up to the nearest clock upper bound, else exponential
+};
+
+struct DevInsn {
+    uint8_t op;
+    uint8_t dst;
+    uint16_t a, b, c;
+};
+
+struct DevLoc {             // 16 bytes
+    uint8_t klass;
+    uint8_t type;
+    uint16_t n_inv;         // clock upper-bound invariants
+    uint32_t rate;          // expression reference
+    uint32_t inv_first;
+    uint32_t edge_first;    // edges [edge_first, edge_first + edge_n)
+};
+
+struct DevInv {             // x <= bound  (strict: x < bound)
+    uint16_t var;
+    uint8_t strict;
+    uint8_t pad;
+    uint32_t bound;
+};
+
+struct DevEdge {            // 32 bytes
+    int32_t dst;
+    int32_t channel;
+    uint32_t weight;
+    uint32_t guard_first;
+    uint32_t upd_first;
+    uint16_t guard_n;
+    uint16_t upd_n;
+    uint8_t dst_goal;       // destination is a goal location
+    uint8_t never;          // has a `!=` guard: never enabled (simulation.cu:249-251)
+    uint16_t pad0;
+    uint32_t pad1;
+};
+
+struct DevGuard {           // variables[var] op rhs
+    uint16_t var;
+    uint8_t op;             // smc_cons_op, never SMC_C_NE
+    uint8_t pad;
+    uint32_t rhs;
+};
+
+struct DevUpd {
+    uint16_t var;
+    uint16_t pad;
+    uint32_t expr;
+};
+
+// Everything the run kernel needs to know about one model; passed by value.
+struct DevModel {
+    const DevLoc *locs;
+    const uint8_t *loc_edge_n;
+    const DevInv *invs;
+    const DevEdge *edges;
+    const DevGuard *guards;
+    const DevUpd *upds;
+    const double *consts;
+    const DevInsn *code;
+    const double *var_init;
+    const int32_t *comp_init_loc;
+    const uint16_t *clocks;     // ids of clock variables
+    int32_t n_comp;
+    int32_t n_vars;
+    int32_t n_clocks;
+    int32_t n_leaves;           // next power of two >= n_comp
+    int32_t n_locs;
+    int32_t n_edges;
+};

@@ -1,0 +1,554 @@
+// sim_kernel.hip — the run kernel and the result reduction for gfx950.
+//
+// Counterpart of cuSMC's simulation_kernel (simulation/simulation.cu:674-901)
+// and its device functions (compute_possible_delay :409-541,
+// find_minimum_delay :544-671, SyncInSerial :357-407, check_edge_enabled
+// :184-266, take_transition :27-181), and of the expression evaluator
+// (simulation/expressions.cu).  Same step semantics, different machine
+// mapping:
+//
+//   * cuSMC runs one block per run and one thread per component, with
+//     ~8 + log2(MC) __syncthreads per step and thread-0 serial sections.
+//     Here a run is a segment of S = 2^k lanes of one wave64 (64/S runs per
+//     wave), and each lane owns C = next_pow2(components)/S component slots
+//     (blocked: lane l holds components l*C .. l*C+C-1).  A workgroup is one
+//     wave, so no workgroup barrier is ever needed; lanes of a segment talk
+//     through cross-lane shuffles, ballots and the segment's LDS block.
+//   * Variables and clocks of each run live in LDS as doubles; clocks and
+//     global time accumulate in FP64.
+//   * The race: cuSMC re-samples every component's delay every step
+//     (simulation.cu:513-540) and takes the minimum.  Here the same law is
+//     sampled with far fewer draws: zero-delay components (branchpoint,
+//     urgent, committed, or a clock bound already reached) win first, lowest
+//     index first; otherwise the exponential components are one
+//     exponential race — minimum delay Exp(sum of rates) and winner chosen in
+//     proportion to its rate — and only components under a clock upper bound
+//     draw their own uniform delay.  The rate sum and the proportional
+//     choice use one fixed pairwise tree over the next power of two of the
+//     component count (in-lane levels, then a butterfly over the segment),
+//     so the result does not depend on the launch shape and a sequential
+//     host implementation reproduces it bit for bit.
+//   * Broadcast receivers compute their enabled sets in parallel and then
+//     fire one at a time in component order (SyncInSerial), each lane
+//     executing its own component's transition.
+//   * Runs are handed out by an atomic counter, so a segment that finishes a
+//     short run immediately starts the next one.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "devmodel.h"
+#include "expr_ops.h"
+#include "kernels.hpp"
+#include "smc_stream.h"
+
+namespace {
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int lowest_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+
+// ------------------------------------------------------------ expressions
+
+__device__ __forceinline__ double fetch_opnd(uint16_t o, double r0, double r1, double r2, double r3,
+                                             const double *vars, const double *consts) {
+    uint32_t k = o >> 14, i = o & 0x3FFFu;
+    if (k == SMC_OPND_REG) return i == 0 ? r0 : i == 1 ? r1 : i == 2 ? r2 : r3;
+    if (k == SMC_OPND_VAR) return vars[i];
+    return consts[i];
+}
+
+// Value of an expression reference (expressions.cu:97-229).
+__device__ double eval_ref(uint32_t ref, const double *vars, const DevModel &m) {
+    uint32_t kind = ref >> 30;
+    if (kind == SMC_REF_CONST) return m.consts[ref & 0x3FFFFFFFu];
+    if (kind == SMC_REF_VAR) return vars[ref & 0x3FFFFFFFu];
+    uint32_t first = ref & 0xFFFFFFu, count = (ref >> 24) & 0x3Fu;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    for (uint32_t k = 0; k < count; ++k) {
+        const DevInsn in = m.code[first + k];
+        double a = fetch_opnd(in.a, r0, r1, r2, r3, vars, m.consts);
+        double b = fetch_opnd(in.b, r0, r1, r2, r3, vars, m.consts);
+        double c = fetch_opnd(in.c, r0, r1, r2, r3, vars, m.consts);
+        double v = smc_apply_op(in.op, a, b, c);
+        r0 = in.dst == 0 ? v : r0;
+        r1 = in.dst == 1 ? v : r1;
+        r2 = in.dst == 2 ? v : r2;
+        r3 = in.dst == 3 ? v : r3;
+    }
+    return r0;
+}
+
+// ------------------------------------------------------------ edges
+
+__device__ __forceinline__ bool guard_holds(int op, double x, double r) {
+    switch (op) {
+    case SMC_C_LE: return x <= r;
+    case SMC_C_LT: return x < r;
+    case SMC_C_GE: return x >= r;
+    case SMC_C_GT: return x > r;
+    case SMC_C_EQ: return fabs(x - r) <= SMC_TOL;
+    default: return false;
+    }
+}
+
+// Enabled edges of location `loc` as a bit mask over its edge list:
+// spontaneous edges (recv_chan == 0) or receive edges on channel recv_chan
+// (check_enabled_edges simulation.cu:268-305, SyncInSerial :363-377).
+__device__ uint64_t enabled_mask(int loc, int recv_chan, const double *vars, const DevModel &m) {
+    const DevLoc L = m.locs[loc];
+    const int ne = m.loc_edge_n[loc];
+    uint64_t mask = 0;
+    for (int i = 0; i < ne; ++i) {
+        const DevEdge &E = m.edges[L.edge_first + i];
+        if (E.never) continue;
+        if (recv_chan ? (E.channel != -recv_chan) : (E.channel < 0)) continue;
+        bool ok = true;
+        for (int g = 0; g < E.guard_n && ok; ++g) {
+            const DevGuard G = m.guards[E.guard_first + g];
+            ok = guard_holds(G.op, vars[G.var], eval_ref(G.rhs, vars, m));
+        }
+        if (ok) mask |= 1ull << i;
+    }
+    return mask;
+}
+
+__device__ __forceinline__ long long edge_weight(int e, const double *vars, const DevModel &m) {
+    int w = smc_to_int(eval_ref(m.edges[e].weight, vars, m));
+    return w > 0 ? w : 0;
+}
+
+// Weighted choice among enabled edges (take_transition simulation.cu:43-79
+// and get_sum_edge_weight :9-24, with the weights of the enabled edges).
+__device__ int pick_edge(uint64_t mask, int edge_first, double u, const double *vars, const DevModel &m) {
+    if ((mask & (mask - 1)) == 0) return lowest_lane(mask);
+    long long W = 0;
+    for (uint64_t r = mask; r; r &= r - 1) W += edge_weight(edge_first + lowest_lane(r), vars, m);
+    long long rnd = (long long)((double)W * u);
+    long long temp = W;
+    for (uint64_t r = mask; r;) {
+        int i = 63 - __clzll((long long)r);
+        r &= ~(1ull << i);
+        long long w = edge_weight(edge_first + i, vars, m);
+        if (temp - w <= rnd) return i;
+        temp -= w;
+    }
+    return lowest_lane(mask);
+}
+
+struct LaneQuery {
+    int qvar;
+    double qmax, qmin;
+};
+
+// Fire edge `e`: updates in order, query variable tracking, new location
+// (take_transition simulation.cu:88-180).  Returns the destination.
+__device__ __forceinline__ int fire_edge(int e, double *vars, const DevModel &m, LaneQuery &q,
+                                         bool &goal, int &chan) {
+    const DevEdge &E = m.edges[e];
+    for (int k = 0; k < E.upd_n; ++k) {
+        const DevUpd U = m.upds[E.upd_first + k];
+        double v = eval_ref(U.expr, vars, m);
+        if ((int)U.var == q.qvar) {
+            if (v > q.qmax) q.qmax = v;
+            if (v < q.qmin) q.qmin = v;
+        }
+        vars[U.var] = v;
+    }
+    if (E.dst_goal) goal = true;
+    if (E.channel > 0) chan = E.channel;
+    return E.dst;
+}
+
+template <int C>
+__device__ __forceinline__ int slot_get(const int (&a)[C], int j) {
+    int r = a[0];
+#pragma unroll
+    for (int k = 1; k < C; ++k) r = (j == k) ? a[k] : r;
+    return r;
+}
+template <int C>
+__device__ __forceinline__ void slot_set(int (&a)[C], int j, int v) {
+#pragma unroll
+    for (int k = 0; k < C; ++k) a[k] = (j == k) ? v : a[k];
+}
+
+// ------------------------------------------------------------ run kernel
+
+template <int C, bool TRACE>
+__global__ __launch_bounds__(64) void sim_kernel(DevModel m, SimParams p) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const int S = p.lanes;
+    const int sl = lane & (S - 1);
+    const int seg_base = lane - sl;
+    const uint64_t seg_mask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << seg_base);
+    double *vars = lds + (size_t)(lane / S) * p.var_stride;
+    const int n = m.n_comp;
+    const double INF = __builtin_huge_val();
+
+    int loc[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) loc[j] = -1;
+
+    bool active = false, done = false;
+    uint64_t run = 0, ridx = 0;
+    double t = 0.0;
+    uint32_t step = 0, trans = 0;
+    int chan = 0, sender = -1;
+    bool goal = false, goal_local = false;
+    uint64_t digest = 0;
+    LaneQuery q{p.qvar, p.qinit, p.qinit};
+
+    for (;;) {
+        if (!active && !done) {
+            unsigned long long r = 0;
+            if (sl == 0) r = atomicAdd(p.counter, 1ull);
+            r = __shfl(r, seg_base);
+            if (r >= p.n_runs) {
+                done = true;
+            } else {
+                ridx = r;
+                run = p.run_begin + r;
+                active = true;
+                t = 0.0; step = 0; trans = 0; chan = 0; sender = -1;
+                goal = false; goal_local = false;
+                q.qmax = p.qinit; q.qmin = p.qinit;
+                digest = TRACE ? smc_mix(0, run) : 0;
+                for (int v = sl; v < m.n_vars; v += S) vars[v] = m.var_init[v];
+#pragma unroll
+                for (int j = 0; j < C; ++j) {
+                    int c = sl * C + j;
+                    loc[j] = c < n ? m.comp_init_loc[c] : -1;
+                }
+                wave_sync();
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        if (done) continue;
+
+        // ---- loop test (simulation.cu:835-849)
+        bool hit = false, end = false;
+        if (!(t < p.bound)) end = true;
+        else if (p.reach && goal) { end = true; hit = true; }
+
+        if (!end) {
+            // ---- pending broadcast: receivers (SyncInSerial simulation.cu:357-407)
+            if (chan > 0) {
+                uint64_t rmask[C];
+                bool anyr = false;
+#pragma unroll
+                for (int j = 0; j < C; ++j) {
+                    int c = sl * C + j;
+                    rmask[j] = (c < n && c != sender) ? enabled_mask(loc[j], chan, vars, m) : 0ull;
+                    anyr |= rmask[j] != 0;
+                }
+                uint64_t bj[C];
+#pragma unroll
+                for (int j = 0; j < C; ++j) bj[j] = __ballot(rmask[j] != 0) & seg_mask;
+                uint64_t lanes_with = __ballot(anyr) & seg_mask;
+                while (lanes_with) {
+                    const int L = lowest_lane(lanes_with);
+                    lanes_with &= lanes_with - 1;
+#pragma unroll
+                    for (int j = 0; j < C; ++j) {
+                        if (!((bj[j] >> L) & 1ull)) continue;
+                        const int c = (L - seg_base) * C + j;
+                        int e_fired = -1;
+                        if (lane == L) {
+                            const int lj = loc[j];
+                            const smc_u32x4 w = smc_draw(p.seed, run, step, SMC_P_RECEIVER, (uint32_t)c);
+                            const int first = (int)m.locs[lj].edge_first;
+                            const int i = pick_edge(rmask[j], first, smc_u01(w.v[0], w.v[1]), vars, m);
+                            int dummy = 0;
+                            e_fired = first + i;
+                            loc[j] = fire_edge(e_fired, vars, m, q, goal_local, dummy);
+                        }
+                        wave_sync();
+                        ++trans;
+                        if (TRACE) {
+                            const int e = __shfl(e_fired, L);
+                            digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)c << 20) | (uint64_t)e);
+                        }
+                    }
+                }
+                chan = 0;
+                sender = -1;
+                goal = (__ballot(goal_local) & seg_mask) != 0;
+            }
+
+            // ---- race (compute_possible_delay :409-541, find_minimum_delay :544-671)
+            double leaf[C];
+            int zero_j = C, unif_j = -1;
+            double umin = INF;
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                leaf[j] = 0.0;
+                const int c = sl * C + j;
+                if (c < n) {
+                    const DevLoc L = m.locs[loc[j]];
+                    if (L.klass == SMC_K_ZERO) {
+                        if (zero_j == C) zero_j = j;
+                    } else {
+                        bool bounded = false;
+                        double M = DBL_MAX;
+                        for (int i = 0; i < L.n_inv; ++i) {
+                            const DevInv iv = m.invs[L.inv_first + i];
+                            double ttb = eval_ref(iv.bound, vars, m) - vars[iv.var];
+                            if (iv.strict) ttb -= 1e-6;
+                            if (ttb >= 0.0) { M = ttb < M ? ttb : M; bounded = true; }
+                        }
+                        if (bounded) {
+                            if (M == 0.0) {
+                                if (zero_j == C) zero_j = j;
+                            } else {
+                                const smc_u32x4 w = smc_draw(p.seed, run, step, SMC_P_UNIFORM, (uint32_t)c);
+                                const double d = M * smc_u01(w.v[0], w.v[1]);
+                                if (d < umin) { umin = d; unif_j = j; }
+                            }
+                        } else {
+                            const double r = eval_ref(L.rate, vars, m);
+                            leaf[j] = r > 0.0 ? r : 0.0;
+                        }
+                    }
+                }
+            }
+
+            double d;
+            int w;
+            const uint64_t zb = __ballot(zero_j < C) & seg_mask;
+            if (zb) {
+                const int Lz = lowest_lane(zb);
+                const int zj = __shfl(zero_j, Lz);
+                w = (Lz - seg_base) * C + zj;
+                d = 0.0;
+            } else {
+                // pairwise tree over the lane's C leaves (heap order, root tr[1])
+                double tr[2 * C];
+#pragma unroll
+                for (int j = 0; j < C; ++j) tr[C + j] = leaf[j];
+#pragma unroll
+                for (int i = C - 1; i >= 1; --i) tr[i] = tr[2 * i] + tr[2 * i + 1];
+                double s = (C == 1) ? leaf[0] : tr[1];
+                double own[6], sib[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    own[k] = s;
+                    sib[k] = 0.0;
+                    if ((1 << k) < S) {
+                        sib[k] = __shfl_xor(s, 1 << k);
+                        s = s + sib[k];
+                    }
+                }
+                const double R = s;
+                double dexp = INF;
+                int wexp = 0x7FFFFFFF;
+                if (R > 0.0) {
+                    const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_RACE, 0);
+                    dexp = -smc_log2(smc_u01(rw.v[0], rw.v[1])) / R;
+                    double v = smc_u01(rw.v[2], rw.v[3]) * R;
+                    int b = 0;
+#pragma unroll
+                    for (int k = 5; k >= 0; --k) {
+                        if ((1 << k) < S) {
+                            const bool in_left = ((sl >> k) & 1) == 0;
+                            const double left = in_left ? own[k] : sib[k];
+                            const double right = in_left ? sib[k] : own[k];
+                            if (!(v < left || right == 0.0)) { v = v - left; b += 1 << k; }
+                        }
+                    }
+                    int node = 1;
+#pragma unroll
+                    for (int lv = 0; (1 << lv) < C; ++lv) {
+                        double left = 0.0, right = 0.0;
+#pragma unroll
+                        for (int cand = (1 << lv); cand < (2 << lv); ++cand)
+                            if (node == cand) { left = tr[2 * cand]; right = tr[2 * cand + 1]; }
+                        if (v < left || right == 0.0) node = 2 * node;
+                        else { v = v - left; node = 2 * node + 1; }
+                    }
+                    const uint64_t wb = __ballot(sl == b) & seg_mask;
+                    const int Lw = lowest_lane(wb);
+                    const int jw = __shfl(node - C, Lw);
+                    wexp = (Lw - seg_base) * C + jw;
+                }
+                double gu = umin;
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                    if ((1 << k) < S) gu = fmin(gu, __shfl_xor(gu, 1 << k));
+                int wu = 0x7FFFFFFF;
+                if (gu < INF) {
+                    const uint64_t ub = __ballot(unif_j >= 0 && umin == gu) & seg_mask;
+                    const int Lu = lowest_lane(ub);
+                    const int ju = __shfl(unif_j, Lu);
+                    wu = (Lu - seg_base) * C + ju;
+                }
+                if (gu < dexp || (gu == dexp && wu < wexp)) { d = gu; w = wu; }
+                else { d = dexp; w = wexp; }
+            }
+
+            if (!(d < INF)) {
+                t = INF;
+                end = true;
+            } else {
+                // clocks advance (find_minimum_delay :632-651)
+                for (int i = sl; i < m.n_clocks; i += S) vars[m.clocks[i]] += d;
+                wave_sync();
+                // winner fires (:654-667)
+                const int wl = seg_base + w / C;
+                const int wj = w % C;
+                int e_fired = -1, new_chan = 0;
+                if (lane == wl) {
+                    const int lw = slot_get<C>(loc, wj);
+                    const uint64_t em = enabled_mask(lw, 0, vars, m);
+                    if (em) {
+                        const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_WINNER, (uint32_t)w);
+                        const int first = (int)m.locs[lw].edge_first;
+                        const int i = pick_edge(em, first, smc_u01(rw.v[0], rw.v[1]), vars, m);
+                        e_fired = first + i;
+                        slot_set<C>(loc, wj, fire_edge(e_fired, vars, m, q, goal_local, new_chan));
+                    }
+                }
+                wave_sync();
+                const int e = __shfl(e_fired, wl);
+                if (e >= 0) {
+                    ++trans;
+                    if (TRACE) digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)w << 20) | (uint64_t)e);
+                    chan = __shfl(new_chan, wl);
+                    sender = chan > 0 ? w : -1;
+                    goal = (__ballot(goal_local) & seg_mask) != 0;
+                }
+                t = t + d;
+                ++step;
+            }
+        }
+
+        if (end) {
+            // per-run result (simulation.cu:887-893, statistics.cuh:126-163)
+            double qmax = q.qmax, qmin = q.qmin;
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if ((1 << k) < S) {
+                    qmax = fmax(qmax, __shfl_xor(qmax, 1 << k));
+                    qmin = fmin(qmin, __shfl_xor(qmin, 1 << k));
+                }
+            const double value = p.is_max ? qmax : qmin;
+            if (TRACE) {
+                digest = smc_mix(digest, smc_bits(t));
+                for (int v = 0; v < m.n_vars; ++v) digest = smc_mix(digest, smc_bits(vars[v]));
+                for (int c = 0; c < n; ++c) {
+                    const int lv = __shfl(slot_get<C>(loc, c % C), seg_base + c / C);
+                    digest = smc_mix(digest, (uint64_t)(uint32_t)lv);
+                }
+            }
+            if (sl == 0) {
+                uint8_t flag;
+                if (p.reach) flag = hit ? 1 : 0;
+                else flag = p.is_max ? (value > p.threshold) : (value < p.threshold);
+                p.flag[ridx] = flag;
+                p.value[ridx] = value;
+                p.steps[ridx] = step;
+                p.trans[ridx] = trans;
+                if (p.time) p.time[ridx] = t;
+                if (TRACE && p.digest) p.digest[ridx] = digest;
+            }
+            wave_sync();
+            active = false;
+        }
+    }
+}
+
+// ------------------------------------------------------------ reduction
+
+// Per-block partial sums over a contiguous slice, fixed tree order.
+__global__ __launch_bounds__(256) void reduce_kernel(const uint8_t *flag, const double *value,
+                                                     const uint32_t *steps, const uint32_t *trans,
+                                                     uint64_t n, uint64_t per_block, ReducePartial *out) {
+    __shared__ double s_sum[256], s_sq[256], s_min[256], s_max[256];
+    __shared__ unsigned long long s_hits[256], s_steps[256], s_trans[256];
+    const uint64_t b0 = (uint64_t)blockIdx.x * per_block;
+    const uint64_t b1 = b0 + per_block < n ? b0 + per_block : n;
+    double sum = 0.0, sq = 0.0, mn = __builtin_huge_val(), mx = -__builtin_huge_val();
+    unsigned long long hits = 0, st = 0, tr = 0;
+    for (uint64_t i = b0 + threadIdx.x; i < b1; i += 256) {
+        const double v = value[i];
+        sum += v;
+        sq = fma(v, v, sq);
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+        hits += flag[i];
+        st += steps[i];
+        tr += trans[i];
+    }
+    s_sum[threadIdx.x] = sum; s_sq[threadIdx.x] = sq; s_min[threadIdx.x] = mn; s_max[threadIdx.x] = mx;
+    s_hits[threadIdx.x] = hits; s_steps[threadIdx.x] = st; s_trans[threadIdx.x] = tr;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s_sum[threadIdx.x] += s_sum[threadIdx.x + o];
+            s_sq[threadIdx.x] += s_sq[threadIdx.x + o];
+            s_min[threadIdx.x] = fmin(s_min[threadIdx.x], s_min[threadIdx.x + o]);
+            s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + o]);
+            s_hits[threadIdx.x] += s_hits[threadIdx.x + o];
+            s_steps[threadIdx.x] += s_steps[threadIdx.x + o];
+            s_trans[threadIdx.x] += s_trans[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ReducePartial r;
+        r.sum = s_sum[0]; r.sumsq = s_sq[0]; r.vmin = s_min[0]; r.vmax = s_max[0];
+        r.hits = s_hits[0]; r.steps = s_steps[0]; r.trans = s_trans[0];
+        out[blockIdx.x] = r;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+
+namespace smc {
+
+template <int C>
+static hipError_t launch_c(const DevModel &m, const SimParams &p, bool trace, int waves, size_t lds, hipStream_t s) {
+    if (trace) hipLaunchKernelGGL((sim_kernel<C, true>), dim3(waves), dim3(64), lds, s, m, p);
+    else hipLaunchKernelGGL((sim_kernel<C, false>), dim3(waves), dim3(64), lds, s, m, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_sim(const DevModel &m, const SimParams &p, int slots, bool trace, int waves,
+                      size_t lds_bytes, hipStream_t stream) {
+    switch (slots) {
+    case 1: return launch_c<1>(m, p, trace, waves, lds_bytes, stream);
+    case 2: return launch_c<2>(m, p, trace, waves, lds_bytes, stream);
+    case 4: return launch_c<4>(m, p, trace, waves, lds_bytes, stream);
+    case 8: return launch_c<8>(m, p, trace, waves, lds_bytes, stream);
+    case 16: return launch_c<16>(m, p, trace, waves, lds_bytes, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t sim_occupancy(int slots, bool trace, size_t lds_bytes, int *blocks_per_cu) {
+    const void *f = nullptr;
+#define SMC_PICK(CC)                                                          \
+    case CC: f = trace ? (const void *)sim_kernel<CC, true> : (const void *)sim_kernel<CC, false>; break;
+    switch (slots) {
+        SMC_PICK(1) SMC_PICK(2) SMC_PICK(4) SMC_PICK(8) SMC_PICK(16)
+    default: return hipErrorInvalidValue;
+    }
+#undef SMC_PICK
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 64, lds_bytes);
+}
+
+hipError_t launch_reduce(const uint8_t *flag, const double *value, const uint32_t *steps,
+                         const uint32_t *trans, uint64_t n, int blocks, ReducePartial *out, hipStream_t s) {
+    uint64_t per_block = (n + blocks - 1) / blocks;
+    if (per_block == 0) per_block = 1;
+    hipLaunchKernelGGL(reduce_kernel, dim3(blocks), dim3(256), 0, s, flag, value, steps, trans, n, per_block, out);
+    return hipGetLastError();
+}
+
+}  // namespace smc

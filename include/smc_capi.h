@@ -1,0 +1,130 @@
+/*
+ * smc_capi.h — C ABI of the MI355X statistical model checker.
+ *
+ * Conventions: C linkage, no exceptions cross this boundary, every function
+ * that can fail returns 0 on success and a negative smc_status otherwise, and
+ * smc_last_error() returns a thread-local message for the last failure.
+ * Handles are not thread-safe; distinct handles may be used concurrently.
+ * The caller owns every host array it passes; the library copies what it
+ * keeps.  Results are on the host when a call returns.
+ */
+#ifndef SMC_CAPI_H
+#define SMC_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "smc_network.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum smc_status {
+    SMC_OK = 0,
+    SMC_E_INVALID = -1,     /* malformed network, query or argument        */
+    SMC_E_PARSE = -2,       /* model file could not be read or understood  */
+    SMC_E_DEVICE = -3,      /* HIP runtime error or no usable GPU          */
+    SMC_E_LIMIT = -4,       /* model exceeds a documented size limit       */
+    SMC_E_ALLOC = -5        /* allocation failed                           */
+};
+
+/* ------------------------------------------------------------ front end */
+
+/* An owned, parsed network plus the names the command line refers to. */
+typedef struct smc_parsed smc_parsed;
+
+/*
+ * Parse a UPPAAL XML model (the reference's uppaal_xml_parser::parse,
+ * automata_parser/uppaal_xml_parser.cpp:623-706, plus the goal marking of
+ * smc.cu:405-436 and domain_optimization_visitor.cpp:48-49,151-167).
+ * loc_query is the `-q` location query "Process.location" or NULL/"" for a
+ * variable query; locations it names, and locations named "Goal", become
+ * goal locations.
+ */
+int smc_parse_uppaal(const char *path, const char *loc_query, smc_parsed **out);
+/* Same, from an in-memory document. */
+int smc_parse_uppaal_string(const char *xml, const char *loc_query, smc_parsed **out);
+/* Borrowed view of the parsed network, valid until smc_parsed_destroy. */
+const smc_network *smc_parsed_network(const smc_parsed *p);
+/* Names: component (template) c, location l, variable v; "" if none. */
+const char *smc_parsed_component_name(const smc_parsed *p, int c);
+const char *smc_parsed_location_name(const smc_parsed *p, int l);
+const char *smc_parsed_variable_name(const smc_parsed *p, int v);
+/* Number of goal locations the query marked (0 means the query matched none). */
+int smc_parsed_goal_count(const smc_parsed *p);
+void smc_parsed_destroy(smc_parsed *p);
+
+/* ------------------------------------------------------------ models */
+
+typedef struct smc_model smc_model;
+
+/*
+ * Build a simulator for a network on GPU `device`: validate it, compile its
+ * expressions to the run kernel's register bytecode, and upload the tables.
+ * Replaces init_shared_model_state (simulation/state/shared_model_state.cuh:
+ * 219-225, shared_model_state.cu:164-689), which allocated one device buffer
+ * per expression and never freed them; this model owns one arena, freed by
+ * smc_model_destroy.
+ */
+int smc_model_create(const smc_network *net, int device, smc_model **out);
+void smc_model_destroy(smc_model *m);
+
+/* Launch shape.  Zero fields mean "choose": lanes_per_run = next power of two
+ * of the component count capped at 64; waves = enough to fill the GPU. */
+typedef struct smc_launch {
+    int32_t lanes_per_run;     /* S: power of two <= 64                    */
+    int32_t waves;             /* persistent waves in the grid              */
+    int32_t waves_per_cu;      /* occupancy target used when waves == 0     */
+    int32_t reserved;
+} smc_launch;
+
+/* Per-run outputs, all optional (NULL = not wanted), each n_runs long. */
+typedef struct smc_run_outputs {
+    uint8_t *flag;         /* reach: goal hit; var probability: beyond threshold */
+    double *value;         /* var query: the per-run max (or min)                */
+    uint32_t *steps;       /* race rounds taken                                  */
+    uint32_t *transitions; /* edges fired, by race winners and receivers         */
+    uint64_t *digest;      /* hash of the whole trajectory (parity checks)       */
+    double *time;          /* global time when the run ended                     */
+} smc_run_outputs;
+
+typedef struct smc_result {
+    uint64_t n_runs;
+    uint64_t hits;          /* reach, var probability: runs counted             */
+    double estimate;        /* hits/n, or the mean per-run value (estimate mode) */
+    double sum, sumsq;      /* of the per-run values (var query)                */
+    double vmin, vmax;      /* of the per-run values (var query)                */
+    double ch_eps;          /* Chernoff-Hoeffding half-width, delta = 0.05       */
+    double clt_half;        /* 95% normal half-width of the estimate             */
+    uint64_t steps;         /* total race rounds                                 */
+    uint64_t transitions;   /* total edges fired                                 */
+    double kernel_ms;       /* run kernel time, HIP events on the model stream   */
+    double reduce_ms;       /* reduction kernel time                             */
+    double wall_ms;         /* whole call                                        */
+    int32_t lanes_per_run;  /* launch actually used                              */
+    int32_t waves;
+} smc_result;
+
+/*
+ * Simulate runs [run_begin, run_begin + n_runs) of `q` with stream `seed`
+ * and reduce them on the GPU.  Counterpart of run_statistical_model_checking
+ * (smc.cu:11-281) with the kernel launch (simulation.cu:674-901) and the
+ * result reduction (simulation/statistics.cuh:126-163, smc.cu:378-401).
+ * Runs are independent and addressed by their global index, so any split of
+ * a run range across calls or devices gives the same per-run results.
+ */
+int smc_run(smc_model *m, const smc_query *q, uint64_t seed,
+            uint64_t run_begin, uint64_t n_runs, const smc_launch *launch,
+            smc_run_outputs *outputs, smc_result *out);
+
+/* Interval helpers (host). */
+double smc_ch_epsilon(uint64_t n, double delta);
+
+const char *smc_last_error(void);
+const char *smc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SMC_CAPI_H */

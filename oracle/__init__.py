@@ -1,0 +1,106 @@
+"""CPU oracle of the simulation semantics — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may
+import this package, and only as the checker or the timed CPU baseline.
+The product (cusmc_amd, libcusmc.so) never calls it.
+
+Parity status of each function (see the header of oracle.cpp for the
+reference file:line each step follows):
+  expression evaluation   pinned: brute force vs numpy/float semantics and
+                          the bytecode interpreter on random trees
+  exponential delay (Q1)  pinned: closed form 1 - 2^(-lambda T)
+  uniform delay (Q2)      pinned: closed form T/U
+  race, tie-break (Q4)    pinned: literal vs canonical sampling agree in law;
+                          winner frequencies lambda_i / sum lambda
+  broadcast order (Q9)    pinned: RaceTest.xml closed form
+  weighted pick (Q10)     pinned: UppaalWeightTest.xml 5:10:1
+  ?: update, null step    pinned: condTest.xml closed form
+  SIR race + broadcast    pinned: exact lumped CTMC (tests/golden)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "liboracle.so"
+
+CANONICAL, LITERAL = 0, 1
+
+
+class OracleOutputs(C.Structure):
+    _fields_ = [("flag", C.POINTER(C.c_uint8)), ("value", C.POINTER(C.c_double)),
+                ("steps", C.POINTER(C.c_uint32)), ("transitions", C.POINTER(C.c_uint32)),
+                ("digest", C.POINTER(C.c_uint64)), ("time", C.POINTER(C.c_double))]
+
+
+class OracleResult(C.Structure):
+    _fields_ = [("n_runs", C.c_uint64), ("hits", C.c_uint64), ("estimate", C.c_double),
+                ("sum", C.c_double), ("sumsq", C.c_double), ("steps", C.c_uint64),
+                ("transitions", C.c_uint64), ("seconds", C.c_double), ("threads", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"oracle not built ({LIB_PATH}); run `python cusmc_amd/build.py oracle`")
+        from cusmc_amd import _capi as A
+        l = C.CDLL(str(LIB_PATH))
+        l.oracle_run.restype = C.c_int
+        l.oracle_run.argtypes = [C.POINTER(A.Network), C.POINTER(A.Query), C.c_uint64, C.c_uint64, C.c_uint64,
+                                 C.c_int, C.c_int, C.POINTER(OracleOutputs), C.POINTER(OracleResult)]
+        l.oracle_trace.restype = C.c_long
+        l.oracle_trace.argtypes = [C.POINTER(A.Network), C.POINTER(A.Query), C.c_uint64, C.c_uint64, C.c_int,
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_long]
+        l.oracle_tree_pick.restype = C.c_double
+        l.oracle_tree_pick.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_double, C.POINTER(C.c_int)]
+        l.oracle_last_error.restype = C.c_char_p
+        _lib = l
+    return _lib
+
+
+def run(network, query, *, seed: int, n_runs: int, run_begin: int = 0, mode: int = CANONICAL,
+        threads: int = 1, per_run: tuple[str, ...] = ()):
+    """Simulate with the oracle; returns (OracleResult, {name: array})."""
+    spec = {"flag": (np.uint8, C.c_uint8), "value": (np.float64, C.c_double),
+            "steps": (np.uint32, C.c_uint32), "transitions": (np.uint32, C.c_uint32),
+            "digest": (np.uint64, C.c_uint64), "time": (np.float64, C.c_double)}
+    outs = OracleOutputs()
+    arrays = {}
+    for name in per_run:
+        dt, ct = spec[name]
+        a = np.zeros(n_runs, dtype=dt)
+        arrays[name] = a
+        setattr(outs, name, a.ctypes.data_as(C.POINTER(ct)))
+    res = OracleResult()
+    rc = lib().oracle_run(network.net_ptr, C.byref(query), C.c_uint64(seed), C.c_uint64(run_begin),
+                          C.c_uint64(n_runs), int(mode), int(threads), C.byref(outs), C.byref(res))
+    if rc != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return res, arrays
+
+
+def trace(network, query, *, seed: int, run: int, mode: int = CANONICAL, cap: int = 100000):
+    ev = np.zeros((cap, 4), dtype=np.int32)
+    tm = np.zeros(cap, dtype=np.float64)
+    n = lib().oracle_trace(network.net_ptr, C.byref(query), C.c_uint64(seed), C.c_uint64(run), int(mode),
+                           ev.ctypes.data_as(C.POINTER(C.c_int32)), tm.ctypes.data_as(C.POINTER(C.c_double)),
+                           cap)
+    if n < 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    n = min(n, cap)
+    return ev[:n], tm[:n]
+
+
+def tree_pick(rates, v: float):
+    r = np.ascontiguousarray(rates, dtype=np.float64)
+    ch = C.c_int()
+    total = lib().oracle_tree_pick(r.ctypes.data_as(C.POINTER(C.c_double)), len(r), float(v), C.byref(ch))
+    return total, ch.value

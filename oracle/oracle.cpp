@@ -1,0 +1,461 @@
+// oracle.cpp — CPU restatement of cuSMC's simulation semantics.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h).  Written independently of the
+// GPU kernel and of the product's bytecode: expressions are evaluated by
+// walking the trees of the smc_network, each run is simulated sequentially.
+//
+// The step, following cuSMC (file:line in the reference):
+//   loop while t < bound                              simulation.cu:835
+//     location query and goal flag set -> hit, stop    simulation.cu:839-849
+//     pending broadcast: every component except the sender collects its
+//       enabled receive edges on the current state, then they fire one
+//       after the other in component order              simulation.cu:357-407
+//     delays: branch/urgent/committed 0                 simulation.cu:438-446
+//       clock upper bounds x<=U (x<U: U-1e-6) with U-x>=0 bound the delay,
+//       uniform on (0, min(U-x)]                        simulation.cu:460-521
+//       otherwise exponential, delay -log2(u)/rate      simulation.cu:522-540
+//     minimum delay, lowest component index on ties     simulation.cu:593-625
+//     clocks advance by it                              simulation.cu:632-651
+//     the winner fires one enabled non-receive edge     simulation.cu:654-667
+//       weighted by integer-truncated weights           simulation.cu:9-79
+//       updates in order, tracking the query variable   simulation.cu:101-134
+//       a goal destination sets the goal flag           simulation.cu:147-168
+//     t += delay                                        simulation.cu:876-882
+//   guards: only conjuncts on a variable; `==` within 1e-5; `!=` disables
+//     the edge                                          simulation.cu:184-266
+//
+// Two ways to sample the race (oracle_mode):
+//   LITERAL   — as cuSMC: every component draws its own delay every step.
+//   CANONICAL — the GPU kernel's sampling of the same law: zero delays win
+//     first (lowest index); the exponential components form one race, total
+//     rate on a fixed pairwise tree, delay -log2(u1)/total, winner by
+//     descending the tree with u2*total; bounded components draw uniforms.
+//     Same Philox stream, same operations, same bits as the GPU.
+// Resolutions of undefined behaviour (by intent): all variables and all
+// clocks are tracked (cuSMC: 3), a receiver picks with its own stream,
+// weights are those of the enabled edges, negative or NaN rates and weights
+// count as 0, `%` by zero gives 0.  FP64 time and clocks (cuSMC: FP32).
+
+#include "oracle.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "smc_stream.h"
+
+namespace {
+
+thread_local std::string g_err;
+const double TOL = 1e-5;  // cuSMC's DBL_EPSILON (include/common_macros.h:72)
+
+int to_int_sat(double x) {
+    if (std::isnan(x)) return 0;
+    if (x >= 2147483647.0) return 2147483647;
+    if (x <= -2147483648.0) return INT32_MIN;
+    return (int)x;
+}
+
+// Tree evaluation with the operator meanings of expressions.cu:10-95.
+double eval(const smc_network &n, int e, const double *vars) {
+    const smc_expr_node &x = n.exprs[e];
+    switch (x.op) {
+    case SMC_E_LIT: return x.value;
+    case SMC_E_VAR: return vars[x.var];
+    case SMC_E_RANDOM: return 1.0;
+    case SMC_E_NEG: return -eval(n, x.a, vars);
+    case SMC_E_SQRT: return std::sqrt(eval(n, x.a, vars));
+    case SMC_E_NOT: return std::fabs(eval(n, x.a, vars)) < TOL ? 1.0 : 0.0;
+    case SMC_E_COND: {
+        double c = eval(n, x.a, vars);
+        return std::fabs(c) > TOL ? eval(n, x.b, vars) : eval(n, x.c, vars);
+    }
+    default: break;
+    }
+    const double a = eval(n, x.a, vars), b = eval(n, x.b, vars);
+    switch (x.op) {
+    case SMC_E_ADD: return a + b;
+    case SMC_E_SUB: return a - b;
+    case SMC_E_MUL: return a * b;
+    case SMC_E_DIV: return a / b;
+    case SMC_E_POW: return smc_pow(a, b);
+    case SMC_E_MOD: {
+        int ia = to_int_sat(a), ib = to_int_sat(b + TOL);
+        if (ib == 0 || (ia == INT32_MIN && ib == -1)) return 0.0;
+        return (double)(ia % ib);
+    }
+    case SMC_E_AND: return (std::fabs(a) > TOL && std::fabs(b) > TOL) ? 1.0 : 0.0;
+    case SMC_E_OR: return (std::fabs(a) > TOL || std::fabs(b) > TOL) ? 1.0 : 0.0;
+    case SMC_E_LE: return a <= b ? 1.0 : 0.0;
+    case SMC_E_GE: return a >= b ? 1.0 : 0.0;
+    case SMC_E_LT: return a < b ? 1.0 : 0.0;
+    case SMC_E_GT: return a > b ? 1.0 : 0.0;
+    case SMC_E_EQ: return std::fabs(a - b) <= TOL ? 1.0 : 0.0;
+    case SMC_E_NE: return std::fabs(a - b) > TOL ? 1.0 : 0.0;
+    default: return 0.0;
+    }
+}
+
+// Canonical pairwise tree: heap t[1..2P), leaves t[P + c].
+struct RaceTree {
+    std::vector<double> t;
+    int P = 1;
+    void build(const std::vector<double> &leaf, int n) {
+        P = 1;
+        while (P < n) P <<= 1;
+        t.assign(2 * P, 0.0);
+        for (int c = 0; c < n; ++c) t[P + c] = leaf[c];
+        for (int i = P - 1; i >= 1; --i) t[i] = t[2 * i] + t[2 * i + 1];
+    }
+    double total() const { return P == 1 ? t[1] : t[1]; }
+    int descend(double v) const {
+        int i = 1;
+        while (i < P) {
+            const double left = t[2 * i], right = t[2 * i + 1];
+            if (v < left || right == 0.0) i = 2 * i;
+            else { v = v - left; i = 2 * i + 1; }
+        }
+        return i - P;
+    }
+};
+
+struct RunOut {
+    bool flag;
+    double value;
+    uint32_t steps, transitions;
+    uint64_t digest;
+    double time;
+};
+
+class Simulator {
+public:
+    Simulator(const smc_network &n, const smc_query &q, uint64_t seed, int mode)
+        : n_(n), q_(q), seed_(seed), mode_(mode) {
+        leaf_.resize(n.n_components);
+        loc_.resize(n.n_components);
+        vars_.resize(n.n_vars > 0 ? n.n_vars : 1);
+        recv_.resize(n.n_components);
+        for (int v = 0; v < n.n_vars; ++v)
+            if (n.var_kind[v] == SMC_VAR_CLOCK) clocks_.push_back(v);
+    }
+
+    // events: optional log of (step, component, edge) per fired edge
+    RunOut run(uint64_t run, std::vector<int32_t> *events = nullptr, std::vector<double> *times = nullptr) {
+        const int N = n_.n_components;
+        for (int v = 0; v < n_.n_vars; ++v) vars_[v] = n_.var_init[v];
+        for (int c = 0; c < N; ++c) loc_[c] = n_.comp_init_loc[c];
+        const bool reach = q_.kind == SMC_QUERY_REACH;
+        const int qvar = reach ? -1 : q_.var;
+        const double qinit = (qvar >= 0 && n_.var_kind[qvar] == SMC_VAR_INT) ? n_.var_init[qvar] : 0.0;
+        qmax_ = qmin_ = qinit;
+        qvar_ = qvar;
+        double t = 0.0;
+        uint32_t step = 0, trans = 0;
+        int chan = 0, sender = -1;
+        bool goal = false, hit = false;
+        uint64_t digest = smc_mix(0, run);
+        auto log = [&](int c, int e) {
+            ++trans;
+            digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)c << 20) | (uint64_t)e);
+            if (events) { events->push_back((int32_t)step); events->push_back(c); events->push_back(e); events->push_back(0); }
+            if (times) times->push_back(t);
+        };
+
+        for (;;) {
+            if (!(t < q_.time_bound)) break;
+            if (reach && goal) { hit = true; break; }
+
+            if (chan > 0) {
+                for (int c = 0; c < N; ++c) {
+                    recv_[c].clear();
+                    if (c != sender) enabled(loc_[c], chan, &recv_[c]);
+                }
+                for (int c = 0; c < N; ++c) {
+                    if (recv_[c].empty()) continue;
+                    smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_RECEIVER, (uint32_t)c);
+                    int e = pick(recv_[c], smc_u01(w.v[0], w.v[1]));
+                    int dummy = 0;
+                    fire(c, e, &goal, &dummy);
+                    log(c, e);
+                }
+                chan = 0;
+                sender = -1;
+            }
+
+            double d;
+            int w;
+            if (mode_ == ORACLE_LITERAL) race_literal(run, step, &d, &w);
+            else race_canonical(run, step, &d, &w);
+            if (!(d < HUGE_VAL)) { t = HUGE_VAL; break; }
+
+            for (int v : clocks_) vars_[v] += d;
+            std::vector<int> en;
+            enabled(loc_[w], 0, &en);
+            if (!en.empty()) {
+                smc_u32x4 r = smc_draw(seed_, run, step, SMC_P_WINNER, (uint32_t)w);
+                int e = pick(en, smc_u01(r.v[0], r.v[1]));
+                int nc = 0;
+                fire(w, e, &goal, &nc);
+                log(w, e);
+                if (nc > 0) { chan = nc; sender = w; }
+            }
+            t = t + d;
+            ++step;
+        }
+
+        RunOut o;
+        o.value = q_.is_max ? qmax_ : qmin_;
+        o.flag = reach ? hit : (q_.is_max ? o.value > q_.threshold : o.value < q_.threshold);
+        o.steps = step;
+        o.transitions = trans;
+        o.time = t;
+        digest = smc_mix(digest, smc_bits(t));
+        for (int v = 0; v < n_.n_vars; ++v) digest = smc_mix(digest, smc_bits(vars_[v]));
+        for (int c = 0; c < N; ++c) digest = smc_mix(digest, (uint64_t)(uint32_t)loc_[c]);
+        o.digest = digest;
+        return o;
+    }
+
+private:
+    const smc_network &n_;
+    smc_query q_;
+    uint64_t seed_;
+    int mode_;
+    std::vector<double> leaf_, vars_;
+    std::vector<int> loc_, clocks_;
+    std::vector<std::vector<int>> recv_;
+    RaceTree tree_;
+    int qvar_ = -1;
+    double qmax_ = 0, qmin_ = 0;
+
+    bool guards_hold(const smc_edge &E) const {
+        for (int g = 0; g < E.guard_count; ++g) {
+            const smc_constraint &G = n_.constraints[E.guard_first + g];
+            if (!G.uses_variable) continue;                  // simulation.cu:209
+            const double x = vars_[G.var];
+            const double r = eval(n_, G.rhs, vars_.data());
+            bool ok;
+            switch (G.op) {
+            case SMC_C_LT: ok = x < r; break;
+            case SMC_C_LE: ok = x <= r; break;
+            case SMC_C_GT: ok = x > r; break;
+            case SMC_C_GE: ok = x >= r; break;
+            case SMC_C_EQ: ok = std::fabs(x - r) <= TOL; break;
+            default: ok = false; break;                      // `!=`: simulation.cu:249-251
+            }
+            if (!ok) return false;
+        }
+        return true;
+    }
+
+    // Enabled edges of location l (global edge indices, file order).
+    void enabled(int l, int recv_chan, std::vector<int> *out) const {
+        const smc_location &L = n_.locations[l];
+        for (int k = 0; k < L.edge_count; ++k) {
+            const int e = L.edge_first + k;
+            const smc_edge &E = n_.edges[e];
+            if (recv_chan ? (E.channel != -recv_chan) : (E.channel < 0)) continue;
+            if (guards_hold(E)) out->push_back(e);
+        }
+    }
+
+    long long weight(int e) const {
+        const smc_edge &E = n_.edges[e];
+        const double w = E.weight >= 0 ? eval(n_, E.weight, vars_.data()) : 1.0;
+        const int wi = to_int_sat(w);
+        return wi > 0 ? wi : 0;
+    }
+
+    int pick(const std::vector<int> &en, double u) const {
+        if (en.size() == 1) return en[0];
+        long long W = 0;
+        for (int e : en) W += weight(e);
+        const long long rnd = (long long)((double)W * u);
+        long long temp = W;
+        for (int i = (int)en.size() - 1; i >= 0; --i) {
+            const long long w = weight(en[i]);
+            if (temp - w <= rnd) return en[i];
+            temp -= w;
+        }
+        return en[0];
+    }
+
+    void fire(int c, int e, bool *goal, int *chan) {
+        const smc_edge &E = n_.edges[e];
+        for (int k = 0; k < E.upd_count; ++k) {
+            const smc_update &U = n_.updates[E.upd_first + k];
+            const double v = eval(n_, U.expr, vars_.data());
+            if (U.var == qvar_) {
+                if (v > qmax_) qmax_ = v;
+                if (v < qmin_) qmin_ = v;
+            }
+            vars_[U.var] = v;
+        }
+        loc_[c] = E.dst;
+        if (n_.locations[E.dst].type == SMC_LOC_GOAL) *goal = true;
+        if (E.channel > 0) *chan = E.channel;
+    }
+
+    // Delay bound of component c from its clock upper-bound invariants;
+    // returns false when nothing bounds it (simulation.cu:460-511).
+    bool bound_of(int c, double *M) const {
+        const smc_location &L = n_.locations[loc_[c]];
+        bool bounded = false;
+        double m = 1.7976931348623157e308;
+        for (int k = 0; k < L.inv_count; ++k) {
+            const smc_constraint &I = n_.constraints[L.inv_first + k];
+            if (!I.uses_variable || n_.var_kind[I.var] != SMC_VAR_CLOCK) continue;
+            if (I.op != SMC_C_LE && I.op != SMC_C_LT) continue;
+            double ttb = eval(n_, I.rhs, vars_.data()) - vars_[I.var];
+            if (I.op == SMC_C_LT) ttb -= 1e-6;
+            if (ttb >= 0.0) { m = ttb < m ? ttb : m; bounded = true; }
+        }
+        *M = m;
+        return bounded;
+    }
+
+    double rate_of(int c) const {
+        const smc_location &L = n_.locations[loc_[c]];
+        return L.rate >= 0 ? eval(n_, L.rate, vars_.data()) : 1.0;
+    }
+
+    void race_canonical(uint64_t run, uint32_t step, double *d_out, int *w_out) {
+        const int N = n_.n_components;
+        int zero = -1, wu = -1;
+        double du = HUGE_VAL;
+        for (int c = 0; c < N; ++c) {
+            leaf_[c] = 0.0;
+            const int type = n_.locations[loc_[c]].type;
+            if (type > SMC_LOC_GOAL) { if (zero < 0) zero = c; continue; }
+            double M;
+            if (bound_of(c, &M)) {
+                if (M == 0.0) { if (zero < 0) zero = c; continue; }
+                smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_UNIFORM, (uint32_t)c);
+                const double d = M * smc_u01(w.v[0], w.v[1]);
+                if (d < du) { du = d; wu = c; }
+            } else {
+                const double r = rate_of(c);
+                leaf_[c] = r > 0.0 ? r : 0.0;
+            }
+        }
+        if (zero >= 0) { *d_out = 0.0; *w_out = zero; return; }
+        tree_.build(leaf_, N);
+        const double R = tree_.t[1];
+        double dexp = HUGE_VAL;
+        int wexp = 0x7FFFFFFF;
+        if (R > 0.0) {
+            smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_RACE, 0);
+            dexp = -smc_log2(smc_u01(w.v[0], w.v[1])) / R;
+            wexp = tree_.descend(smc_u01(w.v[2], w.v[3]) * R);
+        }
+        if (wu < 0) wu = 0x7FFFFFFF;
+        if (du < dexp || (du == dexp && wu < wexp)) { *d_out = du; *w_out = wu; }
+        else { *d_out = dexp; *w_out = wexp; }
+    }
+
+    // cuSMC's own sampling: each component draws (simulation.cu:513-540),
+    // then the minimum with the lowest index on ties (:593-625).
+    void race_literal(uint64_t run, uint32_t step, double *d_out, int *w_out) {
+        const int N = n_.n_components;
+        double best = HUGE_VAL;
+        int win = -1;
+        for (int c = 0; c < N; ++c) {
+            double d;
+            const int type = n_.locations[loc_[c]].type;
+            if (type > SMC_LOC_GOAL) d = 0.0;
+            else {
+                double M;
+                smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_UNIFORM, (uint32_t)c);
+                if (bound_of(c, &M)) d = M * smc_u01(w.v[0], w.v[1]);
+                else {
+                    const double r = rate_of(c);
+                    d = r > 0.0 ? -smc_log2(smc_u01(w.v[0], w.v[1])) / r : HUGE_VAL;
+                }
+            }
+            if (d < best) { best = d; win = c; }
+        }
+        *d_out = best;
+        *w_out = win < 0 ? 0 : win;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char *oracle_last_error(void) { return g_err.c_str(); }
+
+double oracle_tree_pick(const double *rates, int n, double v, int *chosen) {
+    RaceTree t;
+    std::vector<double> leaf(rates, rates + n);
+    t.build(leaf, n);
+    if (chosen) *chosen = t.t[1] > 0.0 ? t.descend(v * t.t[1]) : -1;
+    return t.t[1];
+}
+
+int oracle_run(const smc_network *net, const smc_query *q, uint64_t seed, uint64_t run_begin,
+               uint64_t n_runs, int mode, int threads, oracle_outputs *out, oracle_result *res) {
+    if (!net || !q || !res) { g_err = "null argument"; return -1; }
+    if (q->kind == SMC_QUERY_VAR && (q->var < 0 || q->var >= net->n_vars)) { g_err = "query variable out of range"; return -1; }
+    if (mode != ORACLE_CANONICAL && mode != ORACLE_LITERAL) { g_err = "unknown mode"; return -1; }
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    if ((uint64_t)threads > n_runs) threads = (int)std::max<uint64_t>(1, n_runs);
+    std::vector<RunOut> outs(n_runs);
+    auto t0 = std::chrono::steady_clock::now();
+    auto work = [&](int k) {
+        Simulator sim(*net, *q, seed, mode);
+        const uint64_t b = n_runs * k / threads, e = n_runs * (k + 1) / threads;
+        for (uint64_t i = b; i < e; ++i) outs[i] = sim.run(run_begin + i);
+    };
+    if (threads == 1) work(0);
+    else {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < threads; ++k) pool.emplace_back(work, k);
+        for (auto &th : pool) th.join();
+    }
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    memset(res, 0, sizeof *res);
+    res->n_runs = n_runs;
+    for (uint64_t i = 0; i < n_runs; ++i) {
+        const RunOut &o = outs[i];
+        res->hits += o.flag;
+        res->sum += o.value;
+        res->sumsq += o.value * o.value;
+        res->steps += o.steps;
+        res->transitions += o.transitions;
+        if (out) {
+            if (out->flag) out->flag[i] = o.flag;
+            if (out->value) out->value[i] = o.value;
+            if (out->steps) out->steps[i] = o.steps;
+            if (out->transitions) out->transitions[i] = o.transitions;
+            if (out->digest) out->digest[i] = o.digest;
+            if (out->time) out->time[i] = o.time;
+        }
+    }
+    res->estimate = (q->kind == SMC_QUERY_VAR && q->is_estimate) ? res->sum / (double)n_runs
+                                                                 : (double)res->hits / (double)n_runs;
+    res->seconds = secs;
+    res->threads = threads;
+    return 0;
+}
+
+long oracle_trace(const smc_network *net, const smc_query *q, uint64_t seed, uint64_t run, int mode,
+                  int32_t *events, double *times, long cap) {
+    if (!net || !q) { g_err = "null argument"; return -1; }
+    Simulator sim(*net, *q, seed, mode);
+    std::vector<int32_t> ev;
+    std::vector<double> tm;
+    sim.run(run, &ev, &tm);
+    long n = (long)tm.size();
+    for (long k = 0; k < n && k < cap; ++k) {
+        if (events) for (int j = 0; j < 4; ++j) events[4 * k + j] = ev[4 * k + j];
+        if (times) times[k] = tm[k];
+    }
+    return n;
+}
+
+}  // extern "C"

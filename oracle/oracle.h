@@ -1,0 +1,69 @@
+/*
+ * oracle.h — CPU restatement of cuSMC's simulation semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker of the GPU run
+ * kernel: only the tests, the smoke check and the benchmark's CPU-baseline
+ * leg may load it.  The product (libcusmc) never calls it.
+ *
+ * It walks the expression trees of an smc_network directly (no bytecode) and
+ * runs each simulation sequentially, following simulation/simulation.cu and
+ * simulation/expressions.cu of cuSMC as summarised in oracle.cpp.
+ */
+#ifndef SMC_ORACLE_H
+#define SMC_ORACLE_H
+
+#include <stdint.h>
+#include "smc_network.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum oracle_mode {
+    ORACLE_CANONICAL = 0, /* the GPU's race sampling: same draws, same bits      */
+    ORACLE_LITERAL = 1    /* cuSMC's: every component draws its own delay each step */
+};
+
+typedef struct oracle_outputs {
+    uint8_t *flag;
+    double *value;
+    uint32_t *steps;
+    uint32_t *transitions;
+    uint64_t *digest;
+    double *time;
+} oracle_outputs;
+
+typedef struct oracle_result {
+    uint64_t n_runs;
+    uint64_t hits;
+    double estimate;
+    double sum, sumsq;
+    uint64_t steps, transitions;
+    double seconds;     /* wall time of the simulation loop only */
+    int32_t threads;
+    int32_t reserved;
+} oracle_result;
+
+/* Simulate runs [run_begin, run_begin + n_runs) on `threads` host threads
+ * (0 = all cores).  Returns 0 or -1 (see oracle_last_error). */
+int oracle_run(const smc_network *net, const smc_query *q, uint64_t seed, uint64_t run_begin,
+               uint64_t n_runs, int mode, int threads, oracle_outputs *outputs, oracle_result *res);
+
+/* One run with its event log: events[k] = {step, component, edge, 0} for
+ * every fired edge, times[k] = global time before the step.  Returns the
+ * number of events (may exceed cap; only cap are written) or -1. */
+long oracle_trace(const smc_network *net, const smc_query *q, uint64_t seed, uint64_t run,
+                  int mode, int32_t *events, double *times, long cap);
+
+/* Pairwise-tree sum and proportional descent used by the canonical race,
+ * exposed for tests: returns the tree sum of rates[0..n) padded to the next
+ * power of two, and writes the leaf chosen for target v * sum. */
+double oracle_tree_pick(const double *rates, int n, double v, int *chosen);
+
+const char *oracle_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
