@@ -18,6 +18,7 @@ P = C.POINTER
 QUERY_REACH, QUERY_VAR = 0, 1
 VAR_INT, VAR_CLOCK = 0, 1
 LOC_NORMAL, LOC_GOAL, LOC_BRANCH, LOC_URGENT, LOC_COMMITTED = range(5)
+DEFAULT_MAX_STEPS = 1 << 24
 SMC_OK, SMC_E_INVALID, SMC_E_PARSE, SMC_E_DEVICE, SMC_E_LIMIT, SMC_E_ALLOC = 0, -1, -2, -3, -4, -5
 
 
@@ -53,7 +54,7 @@ class Network(C.Structure):
 
 class Query(C.Structure):
     _fields_ = [("kind", i32), ("var", i32), ("is_max", i32), ("is_estimate", i32),
-                ("threshold", f64), ("time_bound", f64)]
+                ("threshold", f64), ("time_bound", f64), ("max_steps", u32), ("reserved", u32)]
 
 
 class Launch(C.Structure):
@@ -68,7 +69,7 @@ class RunOutputs(C.Structure):
 class Result(C.Structure):
     _fields_ = [("n_runs", u64), ("hits", u64), ("estimate", f64), ("sum", f64), ("sumsq", f64),
                 ("vmin", f64), ("vmax", f64), ("ch_eps", f64), ("clt_half", f64), ("steps", u64),
-                ("transitions", u64), ("kernel_ms", f64), ("reduce_ms", f64), ("wall_ms", f64),
+                ("transitions", u64), ("truncated", u64), ("kernel_ms", f64), ("reduce_ms", f64), ("wall_ms", f64),
                 ("lanes_per_run", i32), ("waves", i32)]
 
 

@@ -275,6 +275,7 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
         p.is_max = q->is_max ? 1 : 0;
         p.lanes = S;
         p.var_stride = stride;
+        p.max_steps = q->max_steps ? q->max_steps : SMC_DEFAULT_MAX_STEPS;
         p.flag = m->d_flag;
         p.value = m->d_value;
         p.steps = m->d_steps;
@@ -289,7 +290,7 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
         hip_check(hipEventRecord(m->ev[1], m->stream), "event");
         int blocks = (int)std::min<uint64_t>((uint64_t)m->n_partials, (n_runs + 4095) / 4096);
         if (blocks < 1) blocks = 1;
-        hip_check(smc::launch_reduce(m->d_flag, m->d_value, m->d_steps, m->d_trans, n_runs, blocks,
+        hip_check(smc::launch_reduce(m->d_flag, m->d_value, m->d_steps, m->d_trans, n_runs, p.max_steps, blocks,
                                      m->d_partials, m->stream), "reduce kernel launch");
         hip_check(hipEventRecord(m->ev[2], m->stream), "event");
         std::vector<ReducePartial> parts(blocks);
@@ -319,6 +320,7 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
             tot.hits += r.hits;
             tot.steps += r.steps;
             tot.trans += r.trans;
+            tot.truncated += r.truncated;
         }
         const double N = (double)n_runs;
         out->n_runs = n_runs;
@@ -329,6 +331,7 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
         out->vmax = tot.vmax;
         out->steps = tot.steps;
         out->transitions = tot.trans;
+        out->truncated = tot.truncated;
         out->ch_eps = smc_ch_epsilon(n_runs, 0.05);
         if (q->kind == SMC_QUERY_VAR && q->is_estimate) {
             out->estimate = tot.sum / N;

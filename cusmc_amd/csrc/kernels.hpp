@@ -17,7 +17,7 @@ struct SimParams {
     int32_t is_max;
     int32_t lanes;              // S, lanes per run (power of two <= 64)
     int32_t var_stride;         // doubles of LDS per segment
-    int32_t pad;
+    uint32_t max_steps;         // runs are cut off after this many race rounds
     uint8_t *flag;              // [n_runs] required
     double *value;              // [n_runs] required
     uint32_t *steps;            // [n_runs] required
@@ -30,7 +30,7 @@ struct SimParams {
 struct ReducePartial {
     double sum, sumsq, vmin, vmax;
     unsigned long long hits, steps, trans;
-    unsigned long long pad;
+    unsigned long long truncated;
 };
 
 namespace smc {
@@ -39,6 +39,7 @@ hipError_t launch_sim(const DevModel &m, const SimParams &p, int slots, bool tra
                       size_t lds_bytes, hipStream_t stream);
 hipError_t sim_occupancy(int slots, bool trace, size_t lds_bytes, int *blocks_per_cu);
 hipError_t launch_reduce(const uint8_t *flag, const double *value, const uint32_t *steps,
-                         const uint32_t *trans, uint64_t n, int blocks, ReducePartial *out, hipStream_t s);
+                         const uint32_t *trans, uint64_t n, uint32_t max_steps, int blocks,
+                         ReducePartial *out, hipStream_t s);
 
 }  // namespace smc

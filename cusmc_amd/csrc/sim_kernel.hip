@@ -234,6 +234,7 @@ __global__ __launch_bounds__(64) void sim_kernel(DevModel m, SimParams p) {
         bool hit = false, end = false;
         if (!(t < p.bound)) end = true;
         else if (p.reach && goal) { end = true; hit = true; }
+        else if (step >= p.max_steps) end = true;
 
         if (!end) {
             // ---- pending broadcast: receivers (SyncInSerial simulation.cu:357-407)
@@ -437,8 +438,8 @@ __global__ __launch_bounds__(64) void sim_kernel(DevModel m, SimParams p) {
                 }
             const double value = p.is_max ? qmax : qmin;
             if (TRACE) {
-                digest = smc_mix(digest, smc_bits(t));
-                for (int v = 0; v < m.n_vars; ++v) digest = smc_mix(digest, smc_bits(vars[v]));
+                digest = smc_mix(digest, smc_digest_bits(t));
+                for (int v = 0; v < m.n_vars; ++v) digest = smc_mix(digest, smc_digest_bits(vars[v]));
                 for (int c = 0; c < n; ++c) {
                     const int lv = __shfl(slot_get<C>(loc, c % C), seg_base + c / C);
                     digest = smc_mix(digest, (uint64_t)(uint32_t)lv);
@@ -466,13 +467,14 @@ __global__ __launch_bounds__(64) void sim_kernel(DevModel m, SimParams p) {
 // Per-block partial sums over a contiguous slice, fixed tree order.
 __global__ __launch_bounds__(256) void reduce_kernel(const uint8_t *flag, const double *value,
                                                      const uint32_t *steps, const uint32_t *trans,
-                                                     uint64_t n, uint64_t per_block, ReducePartial *out) {
+                                                     uint64_t n, uint64_t per_block, uint32_t max_steps,
+                                                     ReducePartial *out) {
     __shared__ double s_sum[256], s_sq[256], s_min[256], s_max[256];
-    __shared__ unsigned long long s_hits[256], s_steps[256], s_trans[256];
+    __shared__ unsigned long long s_hits[256], s_steps[256], s_trans[256], s_trunc[256];
     const uint64_t b0 = (uint64_t)blockIdx.x * per_block;
     const uint64_t b1 = b0 + per_block < n ? b0 + per_block : n;
     double sum = 0.0, sq = 0.0, mn = __builtin_huge_val(), mx = -__builtin_huge_val();
-    unsigned long long hits = 0, st = 0, tr = 0;
+    unsigned long long hits = 0, st = 0, tr = 0, tc = 0;
     for (uint64_t i = b0 + threadIdx.x; i < b1; i += 256) {
         const double v = value[i];
         sum += v;
@@ -482,9 +484,11 @@ __global__ __launch_bounds__(256) void reduce_kernel(const uint8_t *flag, const 
         hits += flag[i];
         st += steps[i];
         tr += trans[i];
+        tc += steps[i] >= max_steps;
     }
     s_sum[threadIdx.x] = sum; s_sq[threadIdx.x] = sq; s_min[threadIdx.x] = mn; s_max[threadIdx.x] = mx;
     s_hits[threadIdx.x] = hits; s_steps[threadIdx.x] = st; s_trans[threadIdx.x] = tr;
+    s_trunc[threadIdx.x] = tc;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
@@ -495,13 +499,14 @@ __global__ __launch_bounds__(256) void reduce_kernel(const uint8_t *flag, const 
             s_hits[threadIdx.x] += s_hits[threadIdx.x + o];
             s_steps[threadIdx.x] += s_steps[threadIdx.x + o];
             s_trans[threadIdx.x] += s_trans[threadIdx.x + o];
+            s_trunc[threadIdx.x] += s_trunc[threadIdx.x + o];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         ReducePartial r;
         r.sum = s_sum[0]; r.sumsq = s_sq[0]; r.vmin = s_min[0]; r.vmax = s_max[0];
-        r.hits = s_hits[0]; r.steps = s_steps[0]; r.trans = s_trans[0];
+        r.hits = s_hits[0]; r.steps = s_steps[0]; r.trans = s_trans[0]; r.truncated = s_trunc[0];
         out[blockIdx.x] = r;
     }
 }
@@ -544,10 +549,11 @@ hipError_t sim_occupancy(int slots, bool trace, size_t lds_bytes, int *blocks_pe
 }
 
 hipError_t launch_reduce(const uint8_t *flag, const double *value, const uint32_t *steps,
-                         const uint32_t *trans, uint64_t n, int blocks, ReducePartial *out, hipStream_t s) {
+                         const uint32_t *trans, uint64_t n, uint32_t max_steps, int blocks,
+                         ReducePartial *out, hipStream_t s) {
     uint64_t per_block = (n + blocks - 1) / blocks;
     if (per_block == 0) per_block = 1;
-    hipLaunchKernelGGL(reduce_kernel, dim3(blocks), dim3(256), 0, s, flag, value, steps, trans, n, per_block, out);
+    hipLaunchKernelGGL(reduce_kernel, dim3(blocks), dim3(256), 0, s, flag, value, steps, trans, n, per_block, max_steps, out);
     return hipGetLastError();
 }
 

@@ -20,7 +20,8 @@
 //     and initial values are truncated to int (shared_model_state.cu:282);
 //   * variable ids follow declaration order, globals first (:228-317).
 // Deliberate differences, each a fix of undefined or crashing behaviour:
-//   trailing // comments are stripped, a declaration keyword is removed only
+//   trailing // comments are stripped, several declarations may share a line,
+//   a declaration keyword is removed only
 //   at the start of the line, constants of one template do not leak into the
 //   next, `c1 && c2 && c3` conditions build a proper chain (build_con at
 //   :526-547 loops on itself), and unknown names are reported as errors.
@@ -460,9 +461,13 @@ private:
     // ---- declarations (declaration_parser.cpp:9-104) ----
     void declare(const std::string &raw, bool global) {
         std::string text = remove_spaces(raw);
+        std::vector<std::string> stmts;
         for (std::string line : split_getline(text, '\n')) {
             size_t cpos = line.find("//");
             if (cpos != std::string::npos) line = line.substr(0, cpos);
+            for (const std::string &st : split_getline(line, ';')) stmts.push_back(st);
+        }
+        for (const std::string &line : stmts) {
             std::string l;
             for (char c : line) if (c != '\t' && c != '\r' && c != '\v' && c != '\f') l += c;
             if (l.empty()) continue;

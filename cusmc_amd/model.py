@@ -78,14 +78,14 @@ class ParsedNetwork:
         return [n.var_kind[i] for i in range(n.n_vars)]
 
 
-def reach_query(time_bound: float) -> A.Query:
-    return A.Query(A.QUERY_REACH, -1, 0, 0, 0.0, float(time_bound))
+def reach_query(time_bound: float, *, max_steps: int = 0) -> A.Query:
+    return A.Query(A.QUERY_REACH, -1, 0, 0, 0.0, float(time_bound), int(max_steps), 0)
 
 
 def var_query(var: int, time_bound: float, *, is_max: bool = True, estimate: bool = False,
-              threshold: float = 0.0) -> A.Query:
+              threshold: float = 0.0, max_steps: int = 0) -> A.Query:
     return A.Query(A.QUERY_VAR, int(var), int(bool(is_max)), int(bool(estimate)), float(threshold),
-                   float(time_bound))
+                   float(time_bound), int(max_steps), 0)
 
 
 @dataclass
@@ -97,6 +97,7 @@ class RunResult:
     clt_half: float
     steps: int
     transitions: int
+    truncated: int
     kernel_ms: float
     reduce_ms: float
     wall_ms: float
@@ -121,13 +122,14 @@ class GpuSimulator:
     """The device model of one network on one GPU (smc_model)."""
 
     def __init__(self, network: ParsedNetwork, device: int = 0):
+        self._h = None
         self.network = network
         h = C.c_void_p()
         A.check(A.lib().smc_model_create(network.net_ptr, int(device), C.byref(h)))
         self._h = h
 
     def close(self) -> None:
-        if self._h and A._lib is not None:
+        if getattr(self, "_h", None) and A._lib is not None:
             A._lib.smc_model_destroy(self._h)
         self._h = None
 
@@ -152,7 +154,7 @@ class GpuSimulator:
         A.check(A.lib().smc_run(self._h, C.byref(query), C.c_uint64(seed), C.c_uint64(run_begin),
                                 C.c_uint64(n_runs), C.byref(launch), C.byref(outs), C.byref(res)))
         return RunResult(res.n_runs, res.hits, res.estimate, res.ch_eps, res.clt_half, res.steps,
-                         res.transitions, res.kernel_ms, res.reduce_ms, res.wall_ms, res.lanes_per_run,
+                         res.transitions, res.truncated, res.kernel_ms, res.reduce_ms, res.wall_ms, res.lanes_per_run,
                          res.waves, res.sum, res.sumsq, res.vmin, res.vmax, arrays)
 
 

@@ -98,6 +98,7 @@ typedef struct smc_result {
     double clt_half;        /* 95% normal half-width of the estimate             */
     uint64_t steps;         /* total race rounds                                 */
     uint64_t transitions;   /* total edges fired                                 */
+    uint64_t truncated;     /* runs cut off at max_steps before the time bound   */
     double kernel_ms;       /* run kernel time, HIP events on the model stream   */
     double reduce_ms;       /* reduction kernel time                             */
     double wall_ms;         /* whole call                                        */

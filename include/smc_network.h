@@ -146,7 +146,13 @@ typedef struct smc_query {
                               0 = fraction of runs beyond threshold         */
     double threshold;      /* SMC_QUERY_VAR probability mode               */
     double time_bound;
+    uint32_t max_steps;    /* race rounds after which a run is cut off (a
+                              Zeno cycle of zero delays never reaches the time
+                              bound); 0 = SMC_DEFAULT_MAX_STEPS              */
+    uint32_t reserved;
 } smc_query;
+
+#define SMC_DEFAULT_MAX_STEPS (1u << 24)
 
 #ifdef __cplusplus
 }

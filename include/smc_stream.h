@@ -176,6 +176,12 @@ SMC_HD double smc_pow(double a, double b) {
     return NAN;
 }
 
+/* Bits of a double for the trace digest: every NaN maps to one pattern
+ * (hosts and GPUs produce NaNs with different sign and payload bits). */
+SMC_HD uint64_t smc_digest_bits(double x) {
+    return x != x ? 0x7FF8000000000000ull : smc_bits(x);
+}
+
 /* Mixing step of the per-run trace digest (splitmix64 finaliser). */
 SMC_HD uint64_t smc_mix(uint64_t h, uint64_t x) {
     uint64_t z = h ^ (x + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2));

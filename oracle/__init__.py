@@ -62,6 +62,16 @@ def lib() -> C.CDLL:
         l.oracle_tree_pick.restype = C.c_double
         l.oracle_tree_pick.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_double, C.POINTER(C.c_int)]
         l.oracle_last_error.restype = C.c_char_p
+        l.oracle_philox.restype = None
+        l.oracle_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        l.oracle_u01.restype = C.c_double
+        l.oracle_u01.argtypes = [C.c_uint32, C.c_uint32]
+        l.oracle_log2.restype = C.c_double
+        l.oracle_log2.argtypes = [C.c_double]
+        l.oracle_pow.restype = C.c_double
+        l.oracle_pow.argtypes = [C.c_double, C.c_double]
+        l.oracle_eval_expr.restype = C.c_double
+        l.oracle_eval_expr.argtypes = [C.POINTER(A.Network), C.c_int, C.POINTER(C.c_double)]
         _lib = l
     return _lib
 
@@ -97,6 +107,19 @@ def trace(network, query, *, seed: int, run: int, mode: int = CANONICAL, cap: in
         raise RuntimeError(lib().oracle_last_error().decode())
     n = min(n, cap)
     return ev[:n], tm[:n]
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().oracle_philox(c, k, o)
+    return tuple(o)
+
+
+def eval_expr(net_ptr, e: int, vars_) -> float:
+    v = np.ascontiguousarray(vars_, dtype=np.float64)
+    return lib().oracle_eval_expr(net_ptr, int(e), v.ctypes.data_as(C.POINTER(C.c_double)))
 
 
 def tree_pick(rates, v: float):

@@ -21,6 +21,8 @@
 //       updates in order, tracking the query variable   simulation.cu:101-134
 //       a goal destination sets the goal flag           simulation.cu:147-168
 //     t += delay                                        simulation.cu:876-882
+//   a run is cut off after max_steps race rounds (no counterpart: cuSMC
+//     spins forever on a cycle of zero delays)
 //   guards: only conjuncts on a variable; `==` within 1e-5; `!=` disables
 //     the edge                                          simulation.cu:184-266
 //
@@ -134,7 +136,8 @@ struct RunOut {
 class Simulator {
 public:
     Simulator(const smc_network &n, const smc_query &q, uint64_t seed, int mode)
-        : n_(n), q_(q), seed_(seed), mode_(mode) {
+        : n_(n), q_(q), seed_(seed), mode_(mode),
+          max_steps_(q.max_steps ? q.max_steps : SMC_DEFAULT_MAX_STEPS) {
         leaf_.resize(n.n_components);
         loc_.resize(n.n_components);
         vars_.resize(n.n_vars > 0 ? n.n_vars : 1);
@@ -168,6 +171,7 @@ public:
         for (;;) {
             if (!(t < q_.time_bound)) break;
             if (reach && goal) { hit = true; break; }
+            if (step >= max_steps_) break;
 
             if (chan > 0) {
                 for (int c = 0; c < N; ++c) {
@@ -213,8 +217,8 @@ public:
         o.steps = step;
         o.transitions = trans;
         o.time = t;
-        digest = smc_mix(digest, smc_bits(t));
-        for (int v = 0; v < n_.n_vars; ++v) digest = smc_mix(digest, smc_bits(vars_[v]));
+        digest = smc_mix(digest, smc_digest_bits(t));
+        for (int v = 0; v < n_.n_vars; ++v) digest = smc_mix(digest, smc_digest_bits(vars_[v]));
         for (int c = 0; c < N; ++c) digest = smc_mix(digest, (uint64_t)(uint32_t)loc_[c]);
         o.digest = digest;
         return o;
@@ -225,6 +229,7 @@ private:
     smc_query q_;
     uint64_t seed_;
     int mode_;
+    uint32_t max_steps_;
     std::vector<double> leaf_, vars_;
     std::vector<int> loc_, clocks_;
     std::vector<std::vector<int>> recv_;
@@ -442,6 +447,17 @@ int oracle_run(const smc_network *net, const smc_query *q, uint64_t seed, uint64
     res->threads = threads;
     return 0;
 }
+
+void oracle_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+    smc_u32x4 c;
+    for (int i = 0; i < 4; ++i) c.v[i] = ctr[i];
+    smc_u32x4 o = smc_philox4x32_10(c, key[0], key[1]);
+    for (int i = 0; i < 4; ++i) out[i] = o.v[i];
+}
+double oracle_u01(uint32_t hi, uint32_t lo) { return smc_u01(hi, lo); }
+double oracle_log2(double x) { return smc_log2(x); }
+double oracle_pow(double a, double b) { return smc_pow(a, b); }
+double oracle_eval_expr(const smc_network *net, int e, const double *vars) { return eval(*net, e, vars); }
 
 long oracle_trace(const smc_network *net, const smc_query *q, uint64_t seed, uint64_t run, int mode,
                   int32_t *events, double *times, long cap) {

@@ -60,6 +60,13 @@ long oracle_trace(const smc_network *net, const smc_query *q, uint64_t seed, uin
  * power of two, and writes the leaf chosen for target v * sum. */
 double oracle_tree_pick(const double *rates, int n, double v, int *chosen);
 
+/* Stream primitives and expression evaluation, exposed for tests. */
+void oracle_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+double oracle_u01(uint32_t hi, uint32_t lo);
+double oracle_log2(double x);
+double oracle_pow(double a, double b);
+double oracle_eval_expr(const smc_network *net, int e, const double *vars);
+
 const char *oracle_last_error(void);
 
 #ifdef __cplusplus
