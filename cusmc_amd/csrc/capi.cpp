@@ -175,7 +175,6 @@ int smc_model_create(const smc_network *net, int device, smc_model **out) {
         struct Part { const void *src; size_t bytes; size_t off; };
         std::vector<Part> parts = {
             {im.locs.data(), im.locs.size() * sizeof(DevLoc), 0},
-            {im.loc_edge_n.data(), im.loc_edge_n.size(), 0},
             {im.invs.data(), im.invs.size() * sizeof(DevInv), 0},
             {im.edges.data(), im.edges.size() * sizeof(DevEdge), 0},
             {im.guards.data(), im.guards.size() * sizeof(DevGuard), 0},
@@ -195,16 +194,15 @@ int smc_model_create(const smc_network *net, int device, smc_model **out) {
         auto at = [&](int i) { return (const void *)((const uint8_t *)m->arena + parts[i].off); };
         DevModel &d = m->dm;
         d.locs = (const DevLoc *)at(0);
-        d.loc_edge_n = (const uint8_t *)at(1);
-        d.invs = (const DevInv *)at(2);
-        d.edges = (const DevEdge *)at(3);
-        d.guards = (const DevGuard *)at(4);
-        d.upds = (const DevUpd *)at(5);
-        d.consts = (const double *)at(6);
-        d.code = (const DevInsn *)at(7);
-        d.var_init = (const double *)at(8);
-        d.comp_init_loc = (const int32_t *)at(9);
-        d.clocks = (const uint16_t *)at(10);
+        d.invs = (const DevInv *)at(1);
+        d.edges = (const DevEdge *)at(2);
+        d.guards = (const DevGuard *)at(3);
+        d.upds = (const DevUpd *)at(4);
+        d.consts = (const double *)at(5);
+        d.code = (const DevInsn *)at(6);
+        d.var_init = (const double *)at(7);
+        d.comp_init_loc = (const int32_t *)at(8);
+        d.clocks = (const uint16_t *)at(9);
         d.n_comp = im.n_comp;
         d.n_vars = im.n_vars;
         int nclk = 0;

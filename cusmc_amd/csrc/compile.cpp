@@ -167,6 +167,10 @@ ModelImage compile_model(const smc_network &n) {
             iv.var = (uint16_t)C.var;
             iv.strict = C.op == SMC_C_LT;
             iv.bound = lw.ref(C.rhs, 0.0);
+            if (SMC_REF_KIND(iv.bound) == SMC_REF_CONST) {
+                iv.bound_const = 1;
+                iv.bound_value = img.consts[iv.bound & 0x3FFFFFFFu];
+            }
             img.invs.push_back(iv);
         }
         size_t ninv = img.invs.size() - d.inv_first;
@@ -176,8 +180,12 @@ ModelImage compile_model(const smc_network &n) {
         if (L.edge_count > SMC_MAX_EDGES_PER_LOC)
             throw std::runtime_error("limit: a location has more than 64 outgoing edges");
         d.edge_first = (uint32_t)L.edge_first;
+        d.edge_n = (uint16_t)L.edge_count;
+        if (SMC_REF_KIND(d.rate) == SMC_REF_CONST) {
+            d.rate_const = 1;
+            d.rate_value = img.consts[d.rate & 0x3FFFFFFFu];
+        }
         img.locs.push_back(d);
-        img.loc_edge_n.push_back((uint8_t)L.edge_count);
     }
     for (int e = 0; e < n.n_edges; ++e) {
         const smc_edge &E = n.edges[e];

@@ -10,7 +10,6 @@ namespace smc {
 
 struct ModelImage {
     std::vector<DevLoc> locs;
-    std::vector<uint8_t> loc_edge_n;
     std::vector<DevInv> invs;
     std::vector<DevEdge> edges;
     std::vector<DevGuard> guards;

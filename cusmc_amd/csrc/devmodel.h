@@ -50,20 +50,26 @@ struct DevInsn {
     uint16_t a, b, c;
 };
 
-struct DevLoc {             // 16 bytes
+struct DevLoc {             // 32 bytes, one pair of 16-byte loads
     uint8_t klass;
     uint8_t type;
     uint16_t n_inv;         // clock upper-bound invariants
     uint32_t rate;          // expression reference
     uint32_t inv_first;
     uint32_t edge_first;    // edges [edge_first, edge_first + edge_n)
+    double rate_value;      // the rate when it is a constant (rate_const)
+    uint16_t edge_n;
+    uint8_t rate_const;
+    uint8_t pad0;
+    uint32_t pad1;
 };
 
-struct DevInv {             // x <= bound  (strict: x < bound)
+struct DevInv {             // x <= bound  (strict: x < bound), 16 bytes
     uint16_t var;
     uint8_t strict;
-    uint8_t pad;
-    uint32_t bound;
+    uint8_t bound_const;    // bound is the constant bound_value
+    uint32_t bound;         // expression reference
+    double bound_value;
 };
 
 struct DevEdge {            // 32 bytes
@@ -96,7 +102,6 @@ struct DevUpd {
 // Everything the run kernel needs to know about one model; passed by value.
 struct DevModel {
     const DevLoc *locs;
-    const uint8_t *loc_edge_n;
     const DevInv *invs;
     const DevEdge *edges;
     const DevGuard *guards;

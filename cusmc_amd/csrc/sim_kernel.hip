@@ -51,35 +51,74 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ int lowest_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
 
-// ------------------------------------------------------------ expressions
-
-__device__ __forceinline__ double fetch_opnd(uint16_t o, double r0, double r1, double r2, double r3,
-                                             const double *vars, const double *consts) {
-    uint32_t k = o >> 14, i = o & 0x3FFFu;
-    if (k == SMC_OPND_REG) return i == 0 ? r0 : i == 1 ? r1 : i == 2 ? r2 : r3;
-    if (k == SMC_OPND_VAR) return vars[i];
-    return consts[i];
+// ------------------------------------------------------------ segment exchange
+//
+// Level k of a butterfly over an aligned segment: each lane receives the
+// value held by the sibling block of 2^k lanes.  Levels 0-1 are DPP quad
+// permutes (lane xor 1, xor 2), levels 2-3 DPP mirrors within 8 and 16
+// lanes, level 4 a swizzle xor 16, level 5 a cross-half permute.  The
+// mirrors reach a lane of the sibling block rather than the xor partner,
+// which gives the same value whenever every lane of a block holds the same
+// value — true at every level of a butterfly reduction.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
 }
 
-// Value of an expression reference (expressions.cu:97-229).
-__device__ double eval_ref(uint32_t ref, const double *vars, const DevModel &m) {
-    uint32_t kind = ref >> 30;
-    if (kind == SMC_REF_CONST) return m.consts[ref & 0x3FFFFFFFu];
-    if (kind == SMC_REF_VAR) return vars[ref & 0x3FFFFFFFu];
-    uint32_t first = ref & 0xFFFFFFu, count = (ref >> 24) & 0x3Fu;
+__device__ __forceinline__ double seg_xchg(double x, int k) {
+    switch (k) {
+    case 0: return dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
+    case 1: return dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
+    case 2: return dpp_f64<0x141>(x);  // row_half_mirror
+    case 3: return dpp_f64<0x140>(x);  // row_mirror
+    case 4: {
+        int lo = __double2loint(x), hi = __double2hiint(x);
+        lo = __builtin_amdgcn_ds_swizzle(lo, 0x401F);  // lane xor 16 within 32
+        hi = __builtin_amdgcn_ds_swizzle(hi, 0x401F);
+        return __hiloint2double(hi, lo);
+    }
+    default: return __shfl_xor(x, 32);
+    }
+}
+
+
+// ------------------------------------------------------------ expressions
+
+// Register-bytecode interpreter (expressions.cu:10-119).  Inlined: an
+// out-of-line call costs the whole kernel registers and time even on models
+// that never reach it.
+__device__ __forceinline__ double eval_code(uint32_t ref, const double *vars, const double *consts,
+                                         const DevInsn *code) {
+    const uint32_t first = ref & 0xFFFFFFu, count = (ref >> 24) & 0x3Fu;
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
     for (uint32_t k = 0; k < count; ++k) {
-        const DevInsn in = m.code[first + k];
-        double a = fetch_opnd(in.a, r0, r1, r2, r3, vars, m.consts);
-        double b = fetch_opnd(in.b, r0, r1, r2, r3, vars, m.consts);
-        double c = fetch_opnd(in.c, r0, r1, r2, r3, vars, m.consts);
-        double v = smc_apply_op(in.op, a, b, c);
+        const DevInsn in = code[first + k];
+        double o[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const uint16_t x = s == 0 ? in.a : s == 1 ? in.b : in.c;
+            const uint32_t kind = x >> 14, i = x & 0x3FFFu;
+            o[s] = kind == SMC_OPND_REG ? (i == 0 ? r0 : i == 1 ? r1 : i == 2 ? r2 : r3)
+                 : kind == SMC_OPND_VAR ? vars[i] : consts[i];
+        }
+        const double v = smc_apply_op(in.op, o[0], o[1], o[2]);
         r0 = in.dst == 0 ? v : r0;
         r1 = in.dst == 1 ? v : r1;
         r2 = in.dst == 2 ? v : r2;
         r3 = in.dst == 3 ? v : r3;
     }
     return r0;
+}
+
+// Value of an expression reference: constants and variables inline.
+__device__ __forceinline__ double eval_ref(uint32_t ref, const double *vars, const DevModel &m) {
+    const uint32_t kind = ref >> 30;
+    if (kind == SMC_REF_CONST) return m.consts[ref & 0x3FFFFFFFu];
+    if (kind == SMC_REF_VAR) return vars[ref & 0x3FFFFFFFu];
+    return eval_code(ref, vars, m.consts, m.code);
 }
 
 // ------------------------------------------------------------ edges
@@ -98,9 +137,10 @@ __device__ __forceinline__ bool guard_holds(int op, double x, double r) {
 // Enabled edges of location `loc` as a bit mask over its edge list:
 // spontaneous edges (recv_chan == 0) or receive edges on channel recv_chan
 // (check_enabled_edges simulation.cu:268-305, SyncInSerial :363-377).
-__device__ uint64_t enabled_mask(int loc, int recv_chan, const double *vars, const DevModel &m) {
-    const DevLoc L = m.locs[loc];
-    const int ne = m.loc_edge_n[loc];
+__device__ __forceinline__ uint64_t enabled_mask(int loc, int recv_chan, const double *vars,
+                                                 const DevModel &m) {
+    const DevLoc &L = m.locs[loc];
+    const int ne = L.edge_n;
     uint64_t mask = 0;
     for (int i = 0; i < ne; ++i) {
         const DevEdge &E = m.edges[L.edge_first + i];
@@ -117,24 +157,29 @@ __device__ uint64_t enabled_mask(int loc, int recv_chan, const double *vars, con
 }
 
 __device__ __forceinline__ long long edge_weight(int e, const double *vars, const DevModel &m) {
-    int w = smc_to_int(eval_ref(m.edges[e].weight, vars, m));
+    const int w = smc_to_int(eval_ref(m.edges[e].weight, vars, m));
     return w > 0 ? w : 0;
 }
 
 // Weighted choice among enabled edges (take_transition simulation.cu:43-79
 // and get_sum_edge_weight :9-24, with the weights of the enabled edges).
-__device__ int pick_edge(uint64_t mask, int edge_first, double u, const double *vars, const DevModel &m) {
+// The draw is made only when there is a choice.
+__device__ __forceinline__ int pick_edge(uint64_t mask, int edge_first, uint64_t seed, uint64_t run,
+                                         uint32_t step, uint32_t purpose, uint32_t comp,
+                                         const double *vars, const DevModel &m) {
     if ((mask & (mask - 1)) == 0) return lowest_lane(mask);
+    const smc_u32x4 w = smc_draw(seed, run, step, purpose, comp);
+    const double u = smc_u01(w.v[0], w.v[1]);
     long long W = 0;
     for (uint64_t r = mask; r; r &= r - 1) W += edge_weight(edge_first + lowest_lane(r), vars, m);
-    long long rnd = (long long)((double)W * u);
+    const long long rnd = (long long)((double)W * u);
     long long temp = W;
     for (uint64_t r = mask; r;) {
-        int i = 63 - __clzll((long long)r);
+        const int i = 63 - __clzll((long long)r);
         r &= ~(1ull << i);
-        long long w = edge_weight(edge_first + i, vars, m);
-        if (temp - w <= rnd) return i;
-        temp -= w;
+        const long long w2 = edge_weight(edge_first + i, vars, m);
+        if (temp - w2 <= rnd) return i;
+        temp -= w2;
     }
     return lowest_lane(mask);
 }
@@ -151,7 +196,7 @@ __device__ __forceinline__ int fire_edge(int e, double *vars, const DevModel &m,
     const DevEdge &E = m.edges[e];
     for (int k = 0; k < E.upd_n; ++k) {
         const DevUpd U = m.upds[E.upd_first + k];
-        double v = eval_ref(U.expr, vars, m);
+        const double v = eval_ref(U.expr, vars, m);
         if ((int)U.var == q.qvar) {
             if (v > q.qmax) q.qmax = v;
             if (v < q.qmin) q.qmin = v;
@@ -163,20 +208,21 @@ __device__ __forceinline__ int fire_edge(int e, double *vars, const DevModel &m,
     return E.dst;
 }
 
-template <int C>
-__device__ __forceinline__ int slot_get(const int (&a)[C], int j) {
-    int r = a[0];
+template <int C, class T>
+__device__ __forceinline__ T slot_get(const T (&a)[C], int j) {
+    T r = a[0];
 #pragma unroll
     for (int k = 1; k < C; ++k) r = (j == k) ? a[k] : r;
     return r;
 }
-template <int C>
-__device__ __forceinline__ void slot_set(int (&a)[C], int j, int v) {
+template <int C, class T>
+__device__ __forceinline__ void slot_set(T (&a)[C], int j, T v) {
 #pragma unroll
     for (int k = 0; k < C; ++k) a[k] = (j == k) ? v : a[k];
 }
 
 // ------------------------------------------------------------ run kernel
+
 
 template <int C, bool TRACE>
 __global__ __launch_bounds__(64) void sim_kernel(DevModel m, SimParams p) {
@@ -221,7 +267,7 @@ __global__ __launch_bounds__(64) void sim_kernel(DevModel m, SimParams p) {
                 for (int v = sl; v < m.n_vars; v += S) vars[v] = m.var_init[v];
 #pragma unroll
                 for (int j = 0; j < C; ++j) {
-                    int c = sl * C + j;
+                    const int c = sl * C + j;
                     loc[j] = c < n ? m.comp_init_loc[c] : -1;
                 }
                 wave_sync();
@@ -237,36 +283,155 @@ __global__ __launch_bounds__(64) void sim_kernel(DevModel m, SimParams p) {
         else if (step >= p.max_steps) end = true;
 
         if (!end) {
-            // ---- pending broadcast: receivers (SyncInSerial simulation.cu:357-407)
-            if (chan > 0) {
-                uint64_t rmask[C];
-                bool anyr = false;
-#pragma unroll
-                for (int j = 0; j < C; ++j) {
-                    int c = sl * C + j;
-                    rmask[j] = (c < n && c != sender) ? enabled_mask(loc[j], chan, vars, m) : 0ull;
-                    anyr |= rmask[j] != 0;
-                }
-                uint64_t bj[C];
-#pragma unroll
-                for (int j = 0; j < C; ++j) bj[j] = __ballot(rmask[j] != 0) & seg_mask;
-                uint64_t lanes_with = __ballot(anyr) & seg_mask;
-                while (lanes_with) {
-                    const int L = lowest_lane(lanes_with);
-                    lanes_with &= lanes_with - 1;
+            double d = 0.0;
+            int w = -1;
+            // phase 0: pending broadcast receivers (SyncInSerial simulation.cu:357-407)
+            // phase 1: race (compute_possible_delay :409-541, find_minimum_delay
+            //          :544-671), clock advance, the winner's transition
+#pragma clang loop unroll(disable)
+            for (int phase = 0; phase < 2; ++phase) {
+                if (phase == 0 && chan <= 0) continue;
+                if (phase == 1) {
+                    double leaf[C], Mb[C];
+                    int zero_j = C;
+                    uint32_t bmask = 0;
 #pragma unroll
                     for (int j = 0; j < C; ++j) {
-                        if (!((bj[j] >> L) & 1ull)) continue;
+                        leaf[j] = 0.0;
+                        Mb[j] = 0.0;
+                        const int c = sl * C + j;
+                        if (c < n) {
+                            const DevLoc L = m.locs[loc[j]];
+                            if (L.klass == SMC_K_ZERO) {
+                                if (zero_j == C) zero_j = j;
+                            } else {
+                                bool bounded = false;
+                                double M = DBL_MAX;
+                                for (int i = 0; i < L.n_inv; ++i) {
+                                    const DevInv iv = m.invs[L.inv_first + i];
+                                    const double bound = iv.bound_const ? iv.bound_value : eval_ref(iv.bound, vars, m);
+                                    double ttb = bound - vars[iv.var];
+                                    if (iv.strict) ttb -= 1e-6;
+                                    if (ttb >= 0.0) { M = ttb < M ? ttb : M; bounded = true; }
+                                }
+                                if (bounded) {
+                                    if (M == 0.0) { if (zero_j == C) zero_j = j; }
+                                    else { Mb[j] = M; bmask |= 1u << j; }
+                                } else {
+                                    const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
+                                    leaf[j] = r > 0.0 ? r : 0.0;
+                                }
+                            }
+                        }
+                    }
+                    const uint64_t zb = __ballot(zero_j < C) & seg_mask;
+                    if (zb) {
+                        const int Lz = lowest_lane(zb);
+                        w = (Lz - seg_base) * C + __shfl(zero_j, Lz);
+                        d = 0.0;
+                    } else {
+                        // uniform delays under clock bounds, lowest slot first on ties
+                        double umin = INF;
+                        int unif_j = -1;
+                        for (uint32_t b = bmask; b; b &= b - 1) {
+                            const int j = __ffs(b) - 1;
+                            const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_UNIFORM, (uint32_t)(sl * C + j));
+                            const double du = slot_get<C>(Mb, j) * smc_u01(rw.v[0], rw.v[1]);
+                            if (du < umin) { umin = du; unif_j = j; }
+                        }
+                        // pairwise tree over the lane's C leaves (heap order, root tr[1])
+                        double tr[2 * C];
+#pragma unroll
+                        for (int j = 0; j < C; ++j) tr[C + j] = leaf[j];
+#pragma unroll
+                        for (int i = C - 1; i >= 1; --i) tr[i] = tr[2 * i] + tr[2 * i + 1];
+                        double s = (C == 1) ? leaf[0] : tr[1];
+                        double own[6], sib[6];
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) {
+                            own[k] = s;
+                            sib[k] = 0.0;
+                            if ((1 << k) < S) {
+                                sib[k] = seg_xchg(s, k);
+                                s = s + sib[k];
+                            }
+                        }
+                        const double R = s;
+                        double dexp = INF;
+                        int wexp = 0x7FFFFFFF;
+                        if (R > 0.0) {
+                            const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_RACE, 0);
+                            dexp = -smc_log2(smc_u01(rw.v[0], rw.v[1])) / R;
+                            double v = smc_u01(rw.v[2], rw.v[3]) * R;
+                            int b = 0;
+#pragma unroll
+                            for (int k = 5; k >= 0; --k) {
+                                if ((1 << k) < S) {
+                                    const bool in_left = ((sl >> k) & 1) == 0;
+                                    const double left = in_left ? own[k] : sib[k];
+                                    const double right = in_left ? sib[k] : own[k];
+                                    if (!(v < left || right == 0.0)) { v = v - left; b += 1 << k; }
+                                }
+                            }
+                            int node = 1;
+#pragma unroll
+                            for (int lv = 0; (1 << lv) < C; ++lv) {
+                                double left = 0.0, right = 0.0;
+#pragma unroll
+                                for (int cand = (1 << lv); cand < (2 << lv); ++cand)
+                                    if (node == cand) { left = tr[2 * cand]; right = tr[2 * cand + 1]; }
+                                if (v < left || right == 0.0) node = 2 * node;
+                                else { v = v - left; node = 2 * node + 1; }
+                            }
+                            const int Lw = lowest_lane(__ballot(sl == b) & seg_mask);
+                            wexp = (Lw - seg_base) * C + __shfl(node - C, Lw);
+                        }
+                        double gu = umin;
+#pragma unroll
+                        for (int k = 0; k < 6; ++k)
+                            if ((1 << k) < S) gu = fmin(gu, seg_xchg(gu, k));
+                        int wu = 0x7FFFFFFF;
+                        if (gu < INF) {
+                            const int Lu = lowest_lane(__ballot(unif_j >= 0 && umin == gu) & seg_mask);
+                            wu = (Lu - seg_base) * C + __shfl(unif_j, Lu);
+                        }
+                        if (gu < dexp || (gu == dexp && wu < wexp)) { d = gu; w = wu; }
+                        else { d = dexp; w = wexp; }
+                    }
+                    if (!(d < INF)) { t = INF; end = true; break; }
+                    // clocks advance (find_minimum_delay :632-651)
+                    for (int i = sl; i < m.n_clocks; i += S) vars[m.clocks[i]] += d;
+                    wave_sync();
+                }
+
+                // enabled edges of the components that may fire in this phase
+                const int recv = phase == 0 ? chan : 0;
+                uint64_t mk[C];
+                bool any = false;
+#pragma unroll
+                for (int j = 0; j < C; ++j) {
+                    const int c = sl * C + j;
+                    const bool sel = phase == 0 ? (c < n && c != sender) : (c == w);
+                    mk[j] = sel ? enabled_mask(loc[j], recv, vars, m) : 0ull;
+                    any |= mk[j] != 0;
+                }
+                // fire them one at a time in component order
+                int new_chan = 0;
+                for (uint64_t lanes_with = __ballot(any) & seg_mask; lanes_with; lanes_with &= lanes_with - 1) {
+                    const int L = lowest_lane(lanes_with);
+#pragma clang loop unroll(disable)
+                    for (int j = 0; j < C; ++j) {
+                        if (!((__ballot(slot_get<C>(mk, j) != 0) >> L) & 1ull)) continue;
                         const int c = (L - seg_base) * C + j;
                         int e_fired = -1;
                         if (lane == L) {
-                            const int lj = loc[j];
-                            const smc_u32x4 w = smc_draw(p.seed, run, step, SMC_P_RECEIVER, (uint32_t)c);
+                            const int lj = slot_get<C>(loc, j);
                             const int first = (int)m.locs[lj].edge_first;
-                            const int i = pick_edge(rmask[j], first, smc_u01(w.v[0], w.v[1]), vars, m);
-                            int dummy = 0;
+                            const int i = pick_edge(slot_get<C>(mk, j), first, p.seed, run, step,
+                                                    phase == 0 ? SMC_P_RECEIVER : SMC_P_WINNER,
+                                                    (uint32_t)c, vars, m);
                             e_fired = first + i;
-                            loc[j] = fire_edge(e_fired, vars, m, q, goal_local, dummy);
+                            slot_set<C>(loc, j, fire_edge(e_fired, vars, m, q, goal_local, new_chan));
                         }
                         wave_sync();
                         ++trans;
@@ -274,157 +439,16 @@ __global__ __launch_bounds__(64) void sim_kernel(DevModel m, SimParams p) {
                             const int e = __shfl(e_fired, L);
                             digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)c << 20) | (uint64_t)e);
                         }
+                        if (phase == 1) {
+                            chan = __shfl(new_chan, L);
+                            sender = chan > 0 ? w : -1;
+                        }
                     }
                 }
-                chan = 0;
-                sender = -1;
+                if (phase == 0) { chan = 0; sender = -1; }
                 goal = (__ballot(goal_local) & seg_mask) != 0;
             }
-
-            // ---- race (compute_possible_delay :409-541, find_minimum_delay :544-671)
-            double leaf[C];
-            int zero_j = C, unif_j = -1;
-            double umin = INF;
-#pragma unroll
-            for (int j = 0; j < C; ++j) {
-                leaf[j] = 0.0;
-                const int c = sl * C + j;
-                if (c < n) {
-                    const DevLoc L = m.locs[loc[j]];
-                    if (L.klass == SMC_K_ZERO) {
-                        if (zero_j == C) zero_j = j;
-                    } else {
-                        bool bounded = false;
-                        double M = DBL_MAX;
-                        for (int i = 0; i < L.n_inv; ++i) {
-                            const DevInv iv = m.invs[L.inv_first + i];
-                            double ttb = eval_ref(iv.bound, vars, m) - vars[iv.var];
-                            if (iv.strict) ttb -= 1e-6;
-                            if (ttb >= 0.0) { M = ttb < M ? ttb : M; bounded = true; }
-                        }
-                        if (bounded) {
-                            if (M == 0.0) {
-                                if (zero_j == C) zero_j = j;
-                            } else {
-                                const smc_u32x4 w = smc_draw(p.seed, run, step, SMC_P_UNIFORM, (uint32_t)c);
-                                const double d = M * smc_u01(w.v[0], w.v[1]);
-                                if (d < umin) { umin = d; unif_j = j; }
-                            }
-                        } else {
-                            const double r = eval_ref(L.rate, vars, m);
-                            leaf[j] = r > 0.0 ? r : 0.0;
-                        }
-                    }
-                }
-            }
-
-            double d;
-            int w;
-            const uint64_t zb = __ballot(zero_j < C) & seg_mask;
-            if (zb) {
-                const int Lz = lowest_lane(zb);
-                const int zj = __shfl(zero_j, Lz);
-                w = (Lz - seg_base) * C + zj;
-                d = 0.0;
-            } else {
-                // pairwise tree over the lane's C leaves (heap order, root tr[1])
-                double tr[2 * C];
-#pragma unroll
-                for (int j = 0; j < C; ++j) tr[C + j] = leaf[j];
-#pragma unroll
-                for (int i = C - 1; i >= 1; --i) tr[i] = tr[2 * i] + tr[2 * i + 1];
-                double s = (C == 1) ? leaf[0] : tr[1];
-                double own[6], sib[6];
-#pragma unroll
-                for (int k = 0; k < 6; ++k) {
-                    own[k] = s;
-                    sib[k] = 0.0;
-                    if ((1 << k) < S) {
-                        sib[k] = __shfl_xor(s, 1 << k);
-                        s = s + sib[k];
-                    }
-                }
-                const double R = s;
-                double dexp = INF;
-                int wexp = 0x7FFFFFFF;
-                if (R > 0.0) {
-                    const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_RACE, 0);
-                    dexp = -smc_log2(smc_u01(rw.v[0], rw.v[1])) / R;
-                    double v = smc_u01(rw.v[2], rw.v[3]) * R;
-                    int b = 0;
-#pragma unroll
-                    for (int k = 5; k >= 0; --k) {
-                        if ((1 << k) < S) {
-                            const bool in_left = ((sl >> k) & 1) == 0;
-                            const double left = in_left ? own[k] : sib[k];
-                            const double right = in_left ? sib[k] : own[k];
-                            if (!(v < left || right == 0.0)) { v = v - left; b += 1 << k; }
-                        }
-                    }
-                    int node = 1;
-#pragma unroll
-                    for (int lv = 0; (1 << lv) < C; ++lv) {
-                        double left = 0.0, right = 0.0;
-#pragma unroll
-                        for (int cand = (1 << lv); cand < (2 << lv); ++cand)
-                            if (node == cand) { left = tr[2 * cand]; right = tr[2 * cand + 1]; }
-                        if (v < left || right == 0.0) node = 2 * node;
-                        else { v = v - left; node = 2 * node + 1; }
-                    }
-                    const uint64_t wb = __ballot(sl == b) & seg_mask;
-                    const int Lw = lowest_lane(wb);
-                    const int jw = __shfl(node - C, Lw);
-                    wexp = (Lw - seg_base) * C + jw;
-                }
-                double gu = umin;
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    if ((1 << k) < S) gu = fmin(gu, __shfl_xor(gu, 1 << k));
-                int wu = 0x7FFFFFFF;
-                if (gu < INF) {
-                    const uint64_t ub = __ballot(unif_j >= 0 && umin == gu) & seg_mask;
-                    const int Lu = lowest_lane(ub);
-                    const int ju = __shfl(unif_j, Lu);
-                    wu = (Lu - seg_base) * C + ju;
-                }
-                if (gu < dexp || (gu == dexp && wu < wexp)) { d = gu; w = wu; }
-                else { d = dexp; w = wexp; }
-            }
-
-            if (!(d < INF)) {
-                t = INF;
-                end = true;
-            } else {
-                // clocks advance (find_minimum_delay :632-651)
-                for (int i = sl; i < m.n_clocks; i += S) vars[m.clocks[i]] += d;
-                wave_sync();
-                // winner fires (:654-667)
-                const int wl = seg_base + w / C;
-                const int wj = w % C;
-                int e_fired = -1, new_chan = 0;
-                if (lane == wl) {
-                    const int lw = slot_get<C>(loc, wj);
-                    const uint64_t em = enabled_mask(lw, 0, vars, m);
-                    if (em) {
-                        const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_WINNER, (uint32_t)w);
-                        const int first = (int)m.locs[lw].edge_first;
-                        const int i = pick_edge(em, first, smc_u01(rw.v[0], rw.v[1]), vars, m);
-                        e_fired = first + i;
-                        slot_set<C>(loc, wj, fire_edge(e_fired, vars, m, q, goal_local, new_chan));
-                    }
-                }
-                wave_sync();
-                const int e = __shfl(e_fired, wl);
-                if (e >= 0) {
-                    ++trans;
-                    if (TRACE) digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)w << 20) | (uint64_t)e);
-                    chan = __shfl(new_chan, wl);
-                    sender = chan > 0 ? w : -1;
-                    goal = (__ballot(goal_local) & seg_mask) != 0;
-                }
-                t = t + d;
-                ++step;
-            }
+            if (!end) { t = t + d; ++step; }
         }
 
         if (end) {
@@ -433,8 +457,8 @@ __global__ __launch_bounds__(64) void sim_kernel(DevModel m, SimParams p) {
 #pragma unroll
             for (int k = 0; k < 6; ++k)
                 if ((1 << k) < S) {
-                    qmax = fmax(qmax, __shfl_xor(qmax, 1 << k));
-                    qmin = fmin(qmin, __shfl_xor(qmin, 1 << k));
+                    qmax = fmax(qmax, seg_xchg(qmax, k));
+                    qmin = fmin(qmin, seg_xchg(qmin, k));
                 }
             const double value = p.is_max ? qmax : qmin;
             if (TRACE) {
