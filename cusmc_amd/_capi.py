@@ -73,6 +73,15 @@ class Result(C.Structure):
                 ("lanes_per_run", i32), ("waves", i32)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("model", C.c_void_p), ("query", Query), ("seed", u64), ("run_begin", u64), ("n_runs", u64)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("graph_ms", f64), ("runs", u64), ("steps", u64), ("transitions", u64), ("groups", i32),
+                ("rounds", i32)]
+
+
 # Every symbol include/smc_capi.h declares, with its signature.
 SIGNATURES = {
     "smc_parse_uppaal": (i32, [C.c_char_p, C.c_char_p, P(C.c_void_p)]),
@@ -86,6 +95,10 @@ SIGNATURES = {
     "smc_model_create": (i32, [P(Network), C.c_int, P(C.c_void_p)]),
     "smc_model_destroy": (None, [C.c_void_p]),
     "smc_run": (i32, [C.c_void_p, P(Query), u64, u64, u64, P(Launch), P(RunOutputs), P(Result)]),
+    "smc_autotune": (i32, [C.c_void_p, P(Query), u64, u64, P(Launch)]),
+    "smc_batch_create": (i32, [P(BatchItem), C.c_int, C.c_int, P(C.c_void_p)]),
+    "smc_batch_run": (i32, [C.c_void_p, C.c_int, P(Result), P(BatchStats)]),
+    "smc_batch_destroy": (None, [C.c_void_p]),
     "smc_ch_epsilon": (f64, [u64, f64]),
     "smc_last_error": (C.c_char_p, []),
     "smc_version": (C.c_char_p, []),

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -56,6 +57,141 @@ struct smc_parsed {
     smc_network view{};
 };
 
+namespace {
+
+// Per-run device outputs of one instance, grown on demand.
+struct RunBuffers {
+    size_t cap = 0;
+    uint8_t *flag = nullptr;
+    double *value = nullptr;
+    uint32_t *steps = nullptr, *trans = nullptr;
+    uint64_t *digest = nullptr;
+    double *time = nullptr;
+
+    void release() {
+        for (void *p : {(void *)flag, (void *)value, (void *)steps, (void *)trans, (void *)digest, (void *)time})
+            if (p) hipFree(p);
+        flag = nullptr; value = nullptr; steps = nullptr; trans = nullptr; digest = nullptr; time = nullptr;
+        cap = 0;
+    }
+    void ensure(size_t n) {
+        if (n <= cap) return;
+        release();
+        size_t c = std::max<size_t>(n, 1024);
+        hip_check(hipMalloc(&flag, c), "hipMalloc flags");
+        hip_check(hipMalloc(&value, c * 8), "hipMalloc values");
+        hip_check(hipMalloc(&steps, c * 4), "hipMalloc steps");
+        hip_check(hipMalloc(&trans, c * 4), "hipMalloc transitions");
+        hip_check(hipMalloc(&digest, c * 8), "hipMalloc digests");
+        hip_check(hipMalloc(&time, c * 8), "hipMalloc times");
+        cap = c;
+    }
+};
+
+// Launch shape for one group of instances.
+struct Plan {
+    int S = 1, C = 1, R = 64, stride = 1;
+    size_t lds = 0;
+    int waves = 1;
+    uint32_t chunk_runs = 64;
+    uint64_t total_chunks = 0;
+};
+
+Plan plan_launch(int P, int max_vars, int lanes, int waves_req, int waves_per_cu, uint64_t total_runs,
+                 uint64_t max_runs_per_inst, bool trace, int cu_count) {
+    Plan pl;
+    int S = lanes ? lanes : (P < 64 ? P : 64);
+    if (S <= 0 || S > 64 || (S & (S - 1))) throw std::runtime_error("lanes_per_run must be a power of two <= 64");
+    if (S > P) S = P;
+    pl.S = S;
+    pl.C = P / S;
+    if (pl.C > 16) throw std::runtime_error("limit: more than 16 components per lane; use more lanes per run");
+    pl.R = 64 / S;
+    pl.stride = (max_vars > 0 ? max_vars : 1) | 1;
+    pl.lds = (size_t)pl.R * pl.stride * sizeof(double);
+    if (pl.lds > 60000) throw std::runtime_error("limit: variables of the packed runs exceed the LDS budget");
+    int waves = waves_req;
+    if (waves <= 0) {
+        int per_cu = 0;
+        hip_check(smc::sim_occupancy(pl.C, trace, pl.lds, &per_cu), "occupancy query");
+        if (waves_per_cu > 0 && waves_per_cu < per_cu) per_cu = waves_per_cu;
+        if (per_cu <= 0) per_cu = 1;
+        waves = cu_count * per_cu;
+    }
+    // chunks: enough per wave to balance uneven run lengths, at most 64 rounds of the wave's segments
+    uint64_t per_wave = (total_runs + (uint64_t)waves - 1) / (uint64_t)waves;
+    uint64_t k = (per_wave + 4ull * pl.R - 1) / (4ull * pl.R);
+    if (k < 1) k = 1;
+    if (k > 64) k = 64;
+    uint64_t chunk = k * pl.R;
+    if (chunk > max_runs_per_inst && max_runs_per_inst > 0) chunk = max_runs_per_inst;
+    if (chunk < 1) chunk = 1;
+    pl.chunk_runs = (uint32_t)chunk;
+    pl.waves = waves;
+    return pl;
+}
+
+void validate_query(const smc_query *q, int n_vars) {
+    if (q->kind != SMC_QUERY_REACH && q->kind != SMC_QUERY_VAR) throw std::runtime_error("unknown query kind");
+    if (q->kind == SMC_QUERY_VAR && (q->var < 0 || q->var >= n_vars))
+        throw std::runtime_error("query variable id out of range");
+    if (!(q->time_bound >= 0.0)) throw std::runtime_error("time bound must be a non-negative number");
+}
+
+SimParams make_params(const smc_query *q, const smc::ModelImage &img, uint64_t seed, uint64_t run_begin,
+                      uint64_t n_runs, const RunBuffers &b, bool want_time, bool trace) {
+    SimParams p{};
+    p.seed = seed;
+    p.run_begin = run_begin;
+    p.n_runs = n_runs;
+    p.bound = q->time_bound;
+    p.threshold = q->threshold;
+    p.reach = q->kind == SMC_QUERY_REACH;
+    p.qvar = q->kind == SMC_QUERY_VAR ? q->var : -1;
+    p.qinit = (p.qvar >= 0 && img.var_kind[p.qvar] == SMC_VAR_INT) ? img.var_init[p.qvar] : 0.0;
+    p.is_max = q->is_max ? 1 : 0;
+    p.max_steps = q->max_steps ? q->max_steps : SMC_DEFAULT_MAX_STEPS;
+    p.flag = b.flag;
+    p.value = b.value;
+    p.steps = b.steps;
+    p.trans = b.trans;
+    p.digest = trace ? b.digest : nullptr;
+    p.time = want_time ? b.time : nullptr;
+    return p;
+}
+
+void fill_result(const smc_query *q, const ReducePartial &t, uint64_t n_runs, smc_result *out) {
+    const double N = (double)n_runs;
+    out->n_runs = n_runs;
+    out->hits = t.hits;
+    out->sum = t.sum;
+    out->sumsq = t.sumsq;
+    out->vmin = t.vmin;
+    out->vmax = t.vmax;
+    out->steps = t.steps;
+    out->transitions = t.trans;
+    out->truncated = t.truncated;
+    out->ch_eps = smc_ch_epsilon(n_runs, 0.05);
+    if (q->kind == SMC_QUERY_VAR && q->is_estimate) {
+        out->estimate = t.sum / N;
+        double var = n_runs > 1 ? std::max(0.0, (t.sumsq - t.sum * t.sum / N) / (N - 1.0)) : 0.0;
+        out->clt_half = 1.959963984540054 * std::sqrt(var / N);
+    } else {
+        double pr = (double)t.hits / N;
+        out->estimate = pr;
+        out->clt_half = 1.959963984540054 * std::sqrt(pr * (1.0 - pr) / N);
+    }
+}
+
+uint32_t reduce_blocks(uint64_t n_runs) {
+    uint64_t b = (n_runs + 4095) / 4096;
+    if (b < 1) b = 1;
+    if (b > 1024) b = 1024;
+    return (uint32_t)b;
+}
+
+}  // namespace
+
 struct smc_model {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -63,44 +199,23 @@ struct smc_model {
     DevModel dm{};
     void *arena = nullptr;
     int cu_count = 0;
-    // per-run device buffers, grown on demand
-    size_t cap = 0;
-    uint8_t *d_flag = nullptr;
-    double *d_value = nullptr;
-    uint32_t *d_steps = nullptr, *d_trans = nullptr;
-    uint64_t *d_digest = nullptr;
-    double *d_time = nullptr;
+    RunBuffers runs;
+    InstDesc h_inst{};
+    InstDesc *d_inst = nullptr;
     unsigned long long *d_counter = nullptr;
-    ReducePartial *d_partials = nullptr;
-    int n_partials = 0;
+    ReducePartial *d_partials = nullptr, *d_acc = nullptr;
+    ReducePartial h_acc{};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    smc_launch tuned{};
+    bool has_tuned = false;
 
     ~smc_model() {
         if (device >= 0) hipSetDevice(device);
-        free_runs();
-        if (d_counter) hipFree(d_counter);
-        if (d_partials) hipFree(d_partials);
-        if (arena) hipFree(arena);
+        runs.release();
+        for (void *p : {(void *)d_inst, (void *)d_counter, (void *)d_partials, (void *)d_acc, arena})
+            if (p) hipFree(p);
         for (auto &e : ev) if (e) hipEventDestroy(e);
         if (stream) hipStreamDestroy(stream);
-    }
-    void free_runs() {
-        for (void *p : {(void *)d_flag, (void *)d_value, (void *)d_steps, (void *)d_trans, (void *)d_digest, (void *)d_time})
-            if (p) hipFree(p);
-        d_flag = nullptr; d_value = nullptr; d_steps = nullptr; d_trans = nullptr; d_digest = nullptr; d_time = nullptr;
-        cap = 0;
-    }
-    void ensure_runs(size_t n) {
-        if (n <= cap) return;
-        free_runs();
-        size_t c = std::max<size_t>(n, 1024);
-        hip_check(hipMalloc(&d_flag, c), "hipMalloc flags");
-        hip_check(hipMalloc(&d_value, c * 8), "hipMalloc values");
-        hip_check(hipMalloc(&d_steps, c * 4), "hipMalloc steps");
-        hip_check(hipMalloc(&d_trans, c * 4), "hipMalloc transitions");
-        hip_check(hipMalloc(&d_digest, c * 8), "hipMalloc digests");
-        hip_check(hipMalloc(&d_time, c * 8), "hipMalloc times");
-        cap = c;
     }
 };
 
@@ -211,9 +326,10 @@ int smc_model_create(const smc_network *net, int device, smc_model **out) {
         d.n_leaves = im.n_leaves;
         d.n_locs = (int)im.locs.size();
         d.n_edges = net->n_edges;
+        hip_check(hipMalloc(&m->d_inst, sizeof(InstDesc)), "hipMalloc instance");
         hip_check(hipMalloc(&m->d_counter, sizeof(unsigned long long)), "hipMalloc counter");
-        m->n_partials = 1024;
-        hip_check(hipMalloc(&m->d_partials, m->n_partials * sizeof(ReducePartial)), "hipMalloc partials");
+        hip_check(hipMalloc(&m->d_partials, 1024 * sizeof(ReducePartial)), "hipMalloc partials");
+        hip_check(hipMalloc(&m->d_acc, sizeof(ReducePartial)), "hipMalloc totals");
         *out = m.release();
         return SMC_OK;
     });
@@ -227,126 +343,304 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
     return guarded([&]() -> int {
         auto wall0 = std::chrono::steady_clock::now();
         memset(out, 0, sizeof *out);
-        if (q->kind != SMC_QUERY_REACH && q->kind != SMC_QUERY_VAR) throw std::runtime_error("unknown query kind");
-        if (q->kind == SMC_QUERY_VAR && (q->var < 0 || q->var >= m->img.n_vars))
-            throw std::runtime_error("query variable id out of range");
-        if (!(q->time_bound >= 0.0)) throw std::runtime_error("time bound must be a non-negative number");
+        validate_query(q, m->img.n_vars);
         if (n_runs == 0) throw std::runtime_error("n_runs must be positive");
-        if (n_runs > (1ull << 32)) throw std::runtime_error("limit: at most 2^32 runs per call");
+        if (n_runs > (1ull << 32) - 1) throw std::runtime_error("limit: at most 2^32-1 runs per call");
         if (run_begin > (~0ull) - n_runs) throw std::runtime_error("run range overflows");
-
-        const int P = m->img.n_leaves;
-        int S = launch && launch->lanes_per_run ? launch->lanes_per_run : (P < 64 ? P : 64);
-        if (S <= 0 || S > 64 || (S & (S - 1))) throw std::runtime_error("lanes_per_run must be a power of two <= 64");
-        if (S > P) S = P;
-        int C = P / S;
-        if (C > 16) throw std::runtime_error("limit: more than 16 components per lane; use more lanes per run");
-        const int R = 64 / S;
-        int stride = m->img.n_vars > 0 ? m->img.n_vars : 1;
-        stride |= 1;
-        size_t lds = (size_t)R * stride * sizeof(double);
-        if (lds > 65536) throw std::runtime_error("limit: variables of the packed runs exceed 64 KiB of LDS");
         const bool trace = outputs && outputs->digest;
-
+        const smc_launch *use = (launch && (launch->lanes_per_run || launch->waves || launch->waves_per_cu))
+                                    ? launch : (m->has_tuned ? &m->tuned : nullptr);
         hip_check(hipSetDevice(m->device), "hipSetDevice");
-        m->ensure_runs(n_runs);
-        int waves = launch && launch->waves > 0 ? launch->waves : 0;
-        if (!waves) {
-            int per_cu = 0;
-            hip_check(smc::sim_occupancy(C, trace, lds, &per_cu), "occupancy query");
-            if (launch && launch->waves_per_cu > 0) per_cu = launch->waves_per_cu;
-            if (per_cu <= 0) per_cu = 1;
-            waves = m->cu_count * per_cu;
-        }
-        uint64_t need = (n_runs + R - 1) / R;
-        if ((uint64_t)waves > need) waves = (int)need;
+        Plan pl = plan_launch(m->img.n_leaves, m->img.n_vars, use ? use->lanes_per_run : 0, use ? use->waves : 0,
+                              use ? use->waves_per_cu : 0, n_runs, n_runs, trace, m->cu_count);
+        m->runs.ensure(n_runs);
+        pl.total_chunks = (n_runs + pl.chunk_runs - 1) / pl.chunk_runs;
+        int waves = pl.waves;
+        if ((uint64_t)waves > pl.total_chunks) waves = (int)pl.total_chunks;
 
-        SimParams p{};
-        p.seed = seed;
-        p.run_begin = run_begin;
-        p.n_runs = n_runs;
-        p.bound = q->time_bound;
-        p.threshold = q->threshold;
-        p.reach = q->kind == SMC_QUERY_REACH;
-        p.qvar = q->kind == SMC_QUERY_VAR ? q->var : -1;
-        p.qinit = (p.qvar >= 0 && m->img.var_kind[p.qvar] == SMC_VAR_INT) ? m->img.var_init[p.qvar] : 0.0;
-        p.is_max = q->is_max ? 1 : 0;
-        p.lanes = S;
-        p.var_stride = stride;
-        p.max_steps = q->max_steps ? q->max_steps : SMC_DEFAULT_MAX_STEPS;
-        p.flag = m->d_flag;
-        p.value = m->d_value;
-        p.steps = m->d_steps;
-        p.trans = m->d_trans;
-        p.digest = trace ? m->d_digest : nullptr;
-        p.time = (outputs && outputs->time) ? m->d_time : nullptr;
-        p.counter = m->d_counter;
+        m->h_inst.m = m->dm;
+        m->h_inst.p = make_params(q, m->img, seed, run_begin, n_runs, m->runs, outputs && outputs->time, trace);
+        m->h_inst.chunk_first = 0;
+        LaunchParams lp{};
+        lp.inst = m->d_inst;
+        lp.n_inst = 1;
+        lp.lanes = pl.S;
+        lp.var_stride = pl.stride;
+        lp.chunk_runs = pl.chunk_runs;
+        lp.total_chunks = pl.total_chunks;
+        lp.counter = m->d_counter;
+        lp.round = nullptr;
+        const uint32_t blocks = reduce_blocks(n_runs);
 
-        hip_check(hipMemsetAsync(m->d_counter, 0, sizeof(unsigned long long), m->stream), "clear counter");
+        hip_check(hipMemcpyAsync(m->d_inst, &m->h_inst, sizeof(InstDesc), hipMemcpyHostToDevice, m->stream), "upload instance");
+        hip_check(smc::launch_prep(m->d_counter, nullptr, 0, m->stream), "prep kernel launch");
         hip_check(hipEventRecord(m->ev[0], m->stream), "event");
-        hip_check(smc::launch_sim(m->dm, p, C, trace, waves, lds, m->stream), "run kernel launch");
+        hip_check(smc::launch_sim(lp, pl.C, trace, waves, pl.lds, m->stream), "run kernel launch");
         hip_check(hipEventRecord(m->ev[1], m->stream), "event");
-        int blocks = (int)std::min<uint64_t>((uint64_t)m->n_partials, (n_runs + 4095) / 4096);
-        if (blocks < 1) blocks = 1;
-        hip_check(smc::launch_reduce(m->d_flag, m->d_value, m->d_steps, m->d_trans, n_runs, p.max_steps, blocks,
-                                     m->d_partials, m->stream), "reduce kernel launch");
+        hip_check(smc::launch_reduce(m->d_inst, 1, blocks, m->d_partials, m->d_acc, 0, m->stream), "reduce kernel launch");
         hip_check(hipEventRecord(m->ev[2], m->stream), "event");
-        std::vector<ReducePartial> parts(blocks);
-        hip_check(hipMemcpyAsync(parts.data(), m->d_partials, blocks * sizeof(ReducePartial),
-                                 hipMemcpyDeviceToHost, m->stream), "copy partials");
+        hip_check(hipMemcpyAsync(&m->h_acc, m->d_acc, sizeof(ReducePartial), hipMemcpyDeviceToHost, m->stream), "copy totals");
         if (outputs) {
-            if (outputs->flag) hip_check(hipMemcpyAsync(outputs->flag, m->d_flag, n_runs, hipMemcpyDeviceToHost, m->stream), "copy flags");
-            if (outputs->value) hip_check(hipMemcpyAsync(outputs->value, m->d_value, n_runs * 8, hipMemcpyDeviceToHost, m->stream), "copy values");
-            if (outputs->steps) hip_check(hipMemcpyAsync(outputs->steps, m->d_steps, n_runs * 4, hipMemcpyDeviceToHost, m->stream), "copy steps");
-            if (outputs->transitions) hip_check(hipMemcpyAsync(outputs->transitions, m->d_trans, n_runs * 4, hipMemcpyDeviceToHost, m->stream), "copy transitions");
-            if (outputs->digest) hip_check(hipMemcpyAsync(outputs->digest, m->d_digest, n_runs * 8, hipMemcpyDeviceToHost, m->stream), "copy digests");
-            if (outputs->time) hip_check(hipMemcpyAsync(outputs->time, m->d_time, n_runs * 8, hipMemcpyDeviceToHost, m->stream), "copy times");
+            const RunBuffers &b = m->runs;
+            if (outputs->flag) hip_check(hipMemcpyAsync(outputs->flag, b.flag, n_runs, hipMemcpyDeviceToHost, m->stream), "copy flags");
+            if (outputs->value) hip_check(hipMemcpyAsync(outputs->value, b.value, n_runs * 8, hipMemcpyDeviceToHost, m->stream), "copy values");
+            if (outputs->steps) hip_check(hipMemcpyAsync(outputs->steps, b.steps, n_runs * 4, hipMemcpyDeviceToHost, m->stream), "copy steps");
+            if (outputs->transitions) hip_check(hipMemcpyAsync(outputs->transitions, b.trans, n_runs * 4, hipMemcpyDeviceToHost, m->stream), "copy transitions");
+            if (outputs->digest) hip_check(hipMemcpyAsync(outputs->digest, b.digest, n_runs * 8, hipMemcpyDeviceToHost, m->stream), "copy digests");
+            if (outputs->time) hip_check(hipMemcpyAsync(outputs->time, b.time, n_runs * 8, hipMemcpyDeviceToHost, m->stream), "copy times");
         }
         hip_check(hipStreamSynchronize(m->stream), "run");
         float k_ms = 0.f, r_ms = 0.f;
         hip_check(hipEventElapsedTime(&k_ms, m->ev[0], m->ev[1]), "event time");
         hip_check(hipEventElapsedTime(&r_ms, m->ev[1], m->ev[2]), "event time");
-
-        ReducePartial tot{};
-        tot.vmin = HUGE_VAL;
-        tot.vmax = -HUGE_VAL;
-        for (const auto &r : parts) {
-            tot.sum += r.sum;
-            tot.sumsq += r.sumsq;
-            tot.vmin = std::fmin(tot.vmin, r.vmin);
-            tot.vmax = std::fmax(tot.vmax, r.vmax);
-            tot.hits += r.hits;
-            tot.steps += r.steps;
-            tot.trans += r.trans;
-            tot.truncated += r.truncated;
-        }
-        const double N = (double)n_runs;
-        out->n_runs = n_runs;
-        out->hits = tot.hits;
-        out->sum = tot.sum;
-        out->sumsq = tot.sumsq;
-        out->vmin = tot.vmin;
-        out->vmax = tot.vmax;
-        out->steps = tot.steps;
-        out->transitions = tot.trans;
-        out->truncated = tot.truncated;
-        out->ch_eps = smc_ch_epsilon(n_runs, 0.05);
-        if (q->kind == SMC_QUERY_VAR && q->is_estimate) {
-            out->estimate = tot.sum / N;
-            double var = n_runs > 1 ? std::max(0.0, (tot.sumsq - tot.sum * tot.sum / N) / (N - 1.0)) : 0.0;
-            out->clt_half = 1.959963984540054 * std::sqrt(var / N);
-        } else {
-            double pr = (double)tot.hits / N;
-            out->estimate = pr;
-            out->clt_half = 1.959963984540054 * std::sqrt(pr * (1.0 - pr) / N);
-        }
+        fill_result(q, m->h_acc, n_runs, out);
         out->kernel_ms = k_ms;
         out->reduce_ms = r_ms;
-        out->lanes_per_run = S;
+        out->lanes_per_run = pl.S;
         out->waves = waves;
         out->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
         return SMC_OK;
     });
 }
+
+int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sample_runs, smc_launch *best) {
+    if (!m || !q) return fail(SMC_E_INVALID, "null argument");
+    return guarded([&]() -> int {
+        if (sample_runs == 0) sample_runs = 65536;
+        const int P = m->img.n_leaves;
+        std::vector<smc_launch> cands;
+        for (int S = P < 64 ? P : 64; S >= 1; S >>= 1) {
+            if (P / S > 16) break;
+            cands.push_back(smc_launch{S, 0, 0, 0});
+        }
+        double best_ms = 1e300;
+        smc_launch pick = cands.front();
+        for (const smc_launch &c : cands) {
+            smc_result r;
+            int rc = smc_run(m, q, seed, 0, sample_runs, &c, nullptr, &r);
+            if (rc != SMC_OK) continue;  // e.g. LDS budget exceeded at this packing
+            rc = smc_run(m, q, seed + 1, sample_runs, sample_runs, &c, nullptr, &r);
+            if (rc == SMC_OK && r.kernel_ms < best_ms) { best_ms = r.kernel_ms; pick = c; }
+        }
+        if (!(best_ms < 1e300)) throw std::runtime_error("no launch shape could run this model");
+        m->tuned = pick;
+        m->has_tuned = true;
+        if (best) *best = pick;
+        return SMC_OK;
+    });
+}
+
+// ------------------------------------------------------------ batches
+
+}  // extern "C"
+
+struct smc_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<hipStream_t> gstreams;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
+    std::vector<smc_batch_item> items;
+    std::vector<RunBuffers> bufs;
+    struct Group {
+        Plan pl;
+        std::vector<int> members;
+        InstDesc *d_inst = nullptr;
+        unsigned long long *d_counter = nullptr;
+        LaunchParams lp{};
+    };
+    std::vector<Group> groups;
+    InstDesc *d_all = nullptr;
+    ReducePartial *d_partials = nullptr, *d_acc = nullptr;
+    uint64_t *d_round = nullptr;
+    uint32_t blocks = 1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~smc_batch() {
+        hipSetDevice(device);
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        for (auto &b : bufs) b.release();
+        for (auto &g : groups) { if (g.d_inst) hipFree(g.d_inst); if (g.d_counter) hipFree(g.d_counter); }
+        for (void *p : {(void *)d_all, (void *)d_partials, (void *)d_acc, (void *)d_round}) if (p) hipFree(p);
+        for (auto e : joins) hipEventDestroy(e);
+        for (hipEvent_t e : {fork, ev0, ev1}) if (e) hipEventDestroy(e);
+        for (auto s : gstreams) hipStreamDestroy(s);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_batch **out) {
+    if (!items || n_items <= 0 || !out) return fail(SMC_E_INVALID, "null argument or empty batch");
+    *out = nullptr;
+    return guarded([&]() -> int {
+        auto b = std::make_unique<smc_batch>();
+        b->items.assign(items, items + n_items);
+        b->device = items[0].model ? items[0].model->device : 0;
+        for (int i = 0; i < n_items; ++i) {
+            const smc_batch_item &it = items[i];
+            if (!it.model) throw std::runtime_error("batch item without a model");
+            if (it.model->device != b->device) throw std::runtime_error("batch items on different devices");
+            validate_query(&it.query, it.model->img.n_vars);
+            if (it.n_runs == 0 || it.n_runs > (1ull << 32) - 1) throw std::runtime_error("batch item n_runs out of range");
+        }
+        hip_check(hipSetDevice(b->device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (hipEvent_t *e : {&b->fork, &b->ev0, &b->ev1}) hip_check(hipEventCreate(e), "hipEventCreate");
+        const int cu = items[0].model->cu_count;
+
+        // group by launch shape: tuned (per size class) or default
+        std::map<std::pair<int, int>, std::vector<int>> by_shape;  // (P, S) -> members
+        std::map<int, int> tuned_S;                                 // P -> S
+        if (tune) {
+            std::map<int, int> rep;  // P -> item with the most components... first seen
+            for (int i = 0; i < n_items; ++i) {
+                const int P = items[i].model->img.n_leaves;
+                auto it = rep.find(P);
+                if (it == rep.end() || items[i].model->img.n_comp > items[it->second].model->img.n_comp) rep[P] = i;
+            }
+            for (auto &kv : rep) {
+                const smc_batch_item &it = items[kv.second];
+                smc_launch best{};
+                uint64_t sample = std::min<uint64_t>(it.n_runs, 16384);
+                if (smc_autotune(it.model, &it.query, it.seed, sample, &best) != SMC_OK)
+                    throw std::runtime_error(std::string("autotune failed: ") + g_last_error);
+                tuned_S[kv.first] = best.lanes_per_run;
+            }
+        }
+        for (int i = 0; i < n_items; ++i) {
+            const smc_model *m = items[i].model;
+            const int P = m->img.n_leaves;
+            int S = tuned_S.count(P) ? tuned_S[P] : (m->has_tuned ? m->tuned.lanes_per_run : 0);
+            if (!S) S = P < 64 ? P : 64;
+            by_shape[{P, S}].push_back(i);
+        }
+        b->bufs.resize(n_items);
+        std::vector<InstDesc> all(n_items);
+        for (auto &kv : by_shape) {
+            smc_batch::Group g;
+            g.members = kv.second;
+            uint64_t total = 0, maxr = 0;
+            int max_vars = 1;
+            for (int i : g.members) {
+                total += items[i].n_runs;
+                maxr = std::max<uint64_t>(maxr, items[i].n_runs);
+                max_vars = std::max(max_vars, items[i].model->img.n_vars);
+            }
+            g.pl = plan_launch(kv.first.first, max_vars, kv.first.second, 0, 0, total, maxr, false, cu);
+            std::vector<InstDesc> inst;
+            uint64_t chunk = 0;
+            for (int i : g.members) {
+                const smc_batch_item &it = items[i];
+                b->bufs[i].ensure(it.n_runs);
+                InstDesc d{};
+                d.m = it.model->dm;
+                d.p = make_params(&it.query, it.model->img, it.seed, it.run_begin, it.n_runs, b->bufs[i], false, false);
+                d.chunk_first = chunk;
+                chunk += (it.n_runs + g.pl.chunk_runs - 1) / g.pl.chunk_runs;
+                inst.push_back(d);
+                all[i] = d;
+            }
+            g.pl.total_chunks = chunk;
+            if ((uint64_t)g.pl.waves > chunk) g.pl.waves = (int)chunk;
+            hip_check(hipMalloc(&g.d_inst, inst.size() * sizeof(InstDesc)), "hipMalloc group");
+            hip_check(hipMemcpy(g.d_inst, inst.data(), inst.size() * sizeof(InstDesc), hipMemcpyHostToDevice), "upload group");
+            hip_check(hipMalloc(&g.d_counter, sizeof(unsigned long long)), "hipMalloc counter");
+            hip_check(hipMemset(g.d_counter, 0, sizeof(unsigned long long)), "clear counter");
+            b->groups.push_back(g);
+        }
+        uint64_t maxr = 0;
+        for (auto &it : b->items) maxr = std::max<uint64_t>(maxr, it.n_runs);
+        b->blocks = std::min<uint32_t>(reduce_blocks(maxr), 64);
+        hip_check(hipMalloc(&b->d_all, n_items * sizeof(InstDesc)), "hipMalloc instances");
+        hip_check(hipMemcpy(b->d_all, all.data(), n_items * sizeof(InstDesc), hipMemcpyHostToDevice), "upload instances");
+        hip_check(hipMalloc(&b->d_partials, (size_t)n_items * b->blocks * sizeof(ReducePartial)), "hipMalloc partials");
+        hip_check(hipMalloc(&b->d_acc, n_items * sizeof(ReducePartial)), "hipMalloc totals");
+        hip_check(hipMalloc(&b->d_round, sizeof(uint64_t)), "hipMalloc round");
+        hip_check(hipMemset(b->d_round, 0, sizeof(uint64_t)), "clear round");
+        for (auto &g : b->groups) {
+            g.lp.inst = g.d_inst;
+            g.lp.n_inst = (int32_t)g.members.size();
+            g.lp.lanes = g.pl.S;
+            g.lp.var_stride = g.pl.stride;
+            g.lp.chunk_runs = g.pl.chunk_runs;
+            g.lp.total_chunks = g.pl.total_chunks;
+            g.lp.counter = g.d_counter;
+            g.lp.round = b->d_round;
+        }
+        b->gstreams.resize(b->groups.size());
+        b->joins.resize(b->groups.size());
+        for (size_t k = 0; k < b->groups.size(); ++k) {
+            hip_check(hipStreamCreateWithFlags(&b->gstreams[k], hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreateWithFlags(&b->joins[k], hipEventDisableTiming), "hipEventCreate");
+        }
+        hip_check(hipDeviceSynchronize(), "batch setup");
+
+        // capture one round: groups run concurrently, then the reduction
+        // accumulates, then the counters are cleared and the round advances
+        hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        hip_check(hipEventRecord(b->fork, b->stream), "fork");
+        for (size_t k = 0; k < b->groups.size(); ++k) {
+            auto &g = b->groups[k];
+            hip_check(hipStreamWaitEvent(b->gstreams[k], b->fork, 0), "fork wait");
+            hip_check(smc::launch_sim(g.lp, g.pl.C, false, g.pl.waves, g.pl.lds, b->gstreams[k]), "capture run kernel");
+            hip_check(hipEventRecord(b->joins[k], b->gstreams[k]), "join");
+            hip_check(hipStreamWaitEvent(b->stream, b->joins[k], 0), "join wait");
+        }
+        hip_check(smc::launch_reduce(b->d_all, n_items, b->blocks, b->d_partials, b->d_acc, 1, b->stream), "capture reduce");
+        for (size_t k = 0; k < b->groups.size(); ++k)
+            hip_check(smc::launch_prep(b->groups[k].d_counter, k == 0 ? b->d_round : nullptr, k == 0 ? 1 : 0, b->stream),
+                      "capture prep");
+        hip_check(hipStreamEndCapture(b->stream, &b->graph), "end capture");
+        hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "instantiate graph");
+        *out = b.release();
+        return SMC_OK;
+    });
+}
+
+int smc_batch_run(smc_batch *b, int rounds, smc_result *results, smc_batch_stats *stats) {
+    if (!b || rounds <= 0) return fail(SMC_E_INVALID, "null batch or non-positive rounds");
+    return guarded([&]() -> int {
+        const int n = (int)b->items.size();
+        hip_check(hipSetDevice(b->device), "hipSetDevice");
+        std::vector<ReducePartial> init(n);
+        for (auto &t : init) { t = ReducePartial{}; t.vmin = HUGE_VAL; t.vmax = -HUGE_VAL; }
+        hip_check(hipMemcpyAsync(b->d_acc, init.data(), n * sizeof(ReducePartial), hipMemcpyHostToDevice, b->stream), "init totals");
+        hip_check(hipEventRecord(b->ev0, b->stream), "event");
+        for (int r = 0; r < rounds; ++r) hip_check(hipGraphLaunch(b->exec, b->stream), "graph launch");
+        hip_check(hipEventRecord(b->ev1, b->stream), "event");
+        std::vector<ReducePartial> acc(n);
+        hip_check(hipMemcpyAsync(acc.data(), b->d_acc, n * sizeof(ReducePartial), hipMemcpyDeviceToHost, b->stream), "copy totals");
+        hip_check(hipStreamSynchronize(b->stream), "batch run");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, b->ev0, b->ev1), "event time");
+        uint64_t runs = 0, steps = 0, trans = 0;
+        for (int i = 0; i < n; ++i) {
+            const uint64_t nr = b->items[i].n_runs * (uint64_t)rounds;
+            runs += nr;
+            steps += acc[i].steps;
+            trans += acc[i].trans;
+            if (results) {
+                memset(&results[i], 0, sizeof(smc_result));
+                fill_result(&b->items[i].query, acc[i], nr, &results[i]);
+                results[i].kernel_ms = ms;
+            }
+        }
+        if (stats) {
+            stats->graph_ms = ms;
+            stats->runs = runs;
+            stats->steps = steps;
+            stats->transitions = trans;
+            stats->groups = (int32_t)b->groups.size();
+            stats->rounds = rounds;
+        }
+        return SMC_OK;
+    });
+}
+
+void smc_batch_destroy(smc_batch *b) { delete b; }
 
 }  // extern "C"

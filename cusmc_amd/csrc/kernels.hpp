@@ -5,18 +5,17 @@
 #include <stdint.h>
 #include "devmodel.h"
 
+// Query, run range and outputs of one instance.
 struct SimParams {
     uint64_t seed;
-    uint64_t run_begin;
-    uint64_t n_runs;
+    uint64_t run_begin;         // run index of local run 0 in round 0
+    uint64_t n_runs;            // runs per round
     double bound;
     double threshold;
     double qinit;               // initial max/min of the query variable
     int32_t reach;              // 1: location query
     int32_t qvar;               // query variable or -1
     int32_t is_max;
-    int32_t lanes;              // S, lanes per run (power of two <= 64)
-    int32_t var_stride;         // doubles of LDS per segment
     uint32_t max_steps;         // runs are cut off after this many race rounds
     uint8_t *flag;              // [n_runs] required
     double *value;              // [n_runs] required
@@ -24,7 +23,25 @@ struct SimParams {
     uint32_t *trans;            // [n_runs] required
     uint64_t *digest;           // [n_runs] optional (trace variant)
     double *time;               // [n_runs] optional
-    unsigned long long *counter;  // runs handed out so far (zeroed by the caller)
+};
+
+// One (model, query, runs) instance of a launch.
+struct InstDesc {
+    DevModel m;
+    SimParams p;
+    uint64_t chunk_first;       // first global chunk of this instance
+    uint64_t pad;
+};
+
+struct LaunchParams {
+    const InstDesc *inst;       // sorted by chunk_first
+    int32_t n_inst;
+    int32_t lanes;              // S, lanes per run (power of two <= 64)
+    int32_t var_stride;         // doubles of LDS per segment (>= every n_vars)
+    uint32_t chunk_runs;        // runs per chunk
+    uint64_t total_chunks;
+    unsigned long long *counter;  // chunks handed out (cleared by launch_prep)
+    const uint64_t *round;      // optional: runs of round r start at run_begin + r * n_runs
 };
 
 struct ReducePartial {
@@ -35,11 +52,10 @@ struct ReducePartial {
 
 namespace smc {
 
-hipError_t launch_sim(const DevModel &m, const SimParams &p, int slots, bool trace, int waves,
-                      size_t lds_bytes, hipStream_t stream);
+hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int waves, size_t lds_bytes, hipStream_t stream);
 hipError_t sim_occupancy(int slots, bool trace, size_t lds_bytes, int *blocks_per_cu);
-hipError_t launch_reduce(const uint8_t *flag, const double *value, const uint32_t *steps,
-                         const uint32_t *trans, uint64_t n, uint32_t max_steps, int blocks,
-                         ReducePartial *out, hipStream_t s);
+hipError_t launch_reduce(const InstDesc *inst, int n_inst, uint32_t blocks_per_inst, ReducePartial *partials,
+                         ReducePartial *acc, int accumulate, hipStream_t s);
+hipError_t launch_prep(unsigned long long *counter, uint64_t *round, int advance, hipStream_t s);
 
 }  // namespace smc

@@ -224,17 +224,42 @@ __device__ __forceinline__ void slot_set(T (&a)[C], int j, T v) {
 // ------------------------------------------------------------ run kernel
 
 
+// One wave at a time takes a chunk of runs of one instance from the launch's
+// queue, so the instance (its tables, query and outputs) is uniform across
+// the wave and lives in scalar registers; the wave's segments then pull the
+// chunk's runs one by one from a counter in LDS.
 template <int C, bool TRACE>
-__global__ __launch_bounds__(64) void sim_kernel(DevModel m, SimParams p) {
+__global__ __launch_bounds__(64) void sim_kernel(LaunchParams lp) {
     extern __shared__ double lds[];
+    __shared__ unsigned int chunk_next;
     const int lane = threadIdx.x;
-    const int S = p.lanes;
+    const int S = lp.lanes;
     const int sl = lane & (S - 1);
     const int seg_base = lane - sl;
     const uint64_t seg_mask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << seg_base);
-    double *vars = lds + (size_t)(lane / S) * p.var_stride;
-    const int n = m.n_comp;
+    double *vars = lds + (size_t)(lane / S) * lp.var_stride;
     const double INF = __builtin_huge_val();
+    const uint64_t round = lp.round ? *lp.round : 0;
+
+  for (;;) {
+    unsigned long long g = 0;
+    if (lane == 0) g = atomicAdd(lp.counter, 1ull);
+    g = __shfl(g, 0);
+    if (g >= lp.total_chunks) break;
+    int lo = 0, hi = lp.n_inst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (lp.inst[mid].chunk_first <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const DevModel m = lp.inst[lo].m;
+    const SimParams p = lp.inst[lo].p;
+    const uint64_t chunk = g - lp.inst[lo].chunk_first;
+    const uint32_t r_first = (uint32_t)(chunk * lp.chunk_runs);
+    const uint32_t r_end = (uint32_t)((uint64_t)r_first + lp.chunk_runs < p.n_runs ? r_first + lp.chunk_runs : p.n_runs);
+    if (lane == 0) chunk_next = r_first;
+    wave_sync();
+    const int n = m.n_comp;
 
     int loc[C];
 #pragma unroll
@@ -251,14 +276,14 @@ __global__ __launch_bounds__(64) void sim_kernel(DevModel m, SimParams p) {
 
     for (;;) {
         if (!active && !done) {
-            unsigned long long r = 0;
-            if (sl == 0) r = atomicAdd(p.counter, 1ull);
+            unsigned int r = 0;
+            if (sl == 0) r = atomicAdd(&chunk_next, 1u);
             r = __shfl(r, seg_base);
-            if (r >= p.n_runs) {
+            if (r >= r_end) {
                 done = true;
             } else {
                 ridx = r;
-                run = p.run_begin + r;
+                run = p.run_begin + round * p.n_runs + r;
                 active = true;
                 t = 0.0; step = 0; trans = 0; chan = 0; sender = -1;
                 goal = false; goal_local = false;
@@ -484,31 +509,34 @@ __global__ __launch_bounds__(64) void sim_kernel(DevModel m, SimParams p) {
             active = false;
         }
     }
+  }
 }
 
 // ------------------------------------------------------------ reduction
 
-// Per-block partial sums over a contiguous slice, fixed tree order.
-__global__ __launch_bounds__(256) void reduce_kernel(const uint8_t *flag, const double *value,
-                                                     const uint32_t *steps, const uint32_t *trans,
-                                                     uint64_t n, uint64_t per_block, uint32_t max_steps,
-                                                     ReducePartial *out) {
+// Per-block partial sums over a contiguous slice of one instance's runs
+// (grid.y = instance), fixed tree order.
+__global__ __launch_bounds__(256) void reduce_kernel(const InstDesc *inst, uint32_t blocks_per_inst,
+                                                     ReducePartial *partials) {
     __shared__ double s_sum[256], s_sq[256], s_min[256], s_max[256];
     __shared__ unsigned long long s_hits[256], s_steps[256], s_trans[256], s_trunc[256];
+    const SimParams p = inst[blockIdx.y].p;
+    const uint64_t n = p.n_runs;
+    const uint64_t per_block = (n + blocks_per_inst - 1) / blocks_per_inst;
     const uint64_t b0 = (uint64_t)blockIdx.x * per_block;
     const uint64_t b1 = b0 + per_block < n ? b0 + per_block : n;
     double sum = 0.0, sq = 0.0, mn = __builtin_huge_val(), mx = -__builtin_huge_val();
     unsigned long long hits = 0, st = 0, tr = 0, tc = 0;
     for (uint64_t i = b0 + threadIdx.x; i < b1; i += 256) {
-        const double v = value[i];
+        const double v = p.value[i];
         sum += v;
         sq = fma(v, v, sq);
         mn = fmin(mn, v);
         mx = fmax(mx, v);
-        hits += flag[i];
-        st += steps[i];
-        tr += trans[i];
-        tc += steps[i] >= max_steps;
+        hits += p.flag[i];
+        st += p.steps[i];
+        tr += p.trans[i];
+        tc += p.steps[i] >= p.max_steps;
     }
     s_sum[threadIdx.x] = sum; s_sq[threadIdx.x] = sq; s_min[threadIdx.x] = mn; s_max[threadIdx.x] = mx;
     s_hits[threadIdx.x] = hits; s_steps[threadIdx.x] = st; s_trans[threadIdx.x] = tr;
@@ -531,7 +559,36 @@ __global__ __launch_bounds__(256) void reduce_kernel(const uint8_t *flag, const 
         ReducePartial r;
         r.sum = s_sum[0]; r.sumsq = s_sq[0]; r.vmin = s_min[0]; r.vmax = s_max[0];
         r.hits = s_hits[0]; r.steps = s_steps[0]; r.trans = s_trans[0]; r.truncated = s_trunc[0];
-        out[blockIdx.x] = r;
+        partials[(size_t)blockIdx.y * blocks_per_inst + blockIdx.x] = r;
+    }
+}
+
+// One thread per instance: fold its partials in order into the running
+// totals (a fixed order, so totals do not depend on timing).
+__global__ void finalize_kernel(const ReducePartial *partials, int n_inst, uint32_t blocks_per_inst,
+                                ReducePartial *acc, int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    ReducePartial t;
+    if (accumulate) t = acc[i];
+    else { t.sum = 0; t.sumsq = 0; t.vmin = __builtin_huge_val(); t.vmax = -__builtin_huge_val();
+           t.hits = 0; t.steps = 0; t.trans = 0; t.truncated = 0; }
+    for (uint32_t b = 0; b < blocks_per_inst; ++b) {
+        const ReducePartial r = partials[(size_t)i * blocks_per_inst + b];
+        t.sum += r.sum; t.sumsq += r.sumsq;
+        t.vmin = fmin(t.vmin, r.vmin); t.vmax = fmax(t.vmax, r.vmax);
+        t.hits += r.hits; t.steps += r.steps; t.trans += r.trans; t.truncated += r.truncated;
+    }
+    acc[i] = t;
+}
+
+// Start of a launch or of a captured round: clear the work counter and
+// optionally advance the round (a kernel, not a memset, so it can be
+// recorded in a graph and replayed).
+__global__ void prep_kernel(unsigned long long *counter, uint64_t *round, int advance) {
+    if (threadIdx.x == 0) {
+        *counter = 0;
+        if (round && advance) *round += 1;
     }
 }
 
@@ -542,20 +599,19 @@ __global__ __launch_bounds__(256) void reduce_kernel(const uint8_t *flag, const 
 namespace smc {
 
 template <int C>
-static hipError_t launch_c(const DevModel &m, const SimParams &p, bool trace, int waves, size_t lds, hipStream_t s) {
-    if (trace) hipLaunchKernelGGL((sim_kernel<C, true>), dim3(waves), dim3(64), lds, s, m, p);
-    else hipLaunchKernelGGL((sim_kernel<C, false>), dim3(waves), dim3(64), lds, s, m, p);
+static hipError_t launch_c(const LaunchParams &lp, bool trace, int waves, size_t lds, hipStream_t s) {
+    if (trace) hipLaunchKernelGGL((sim_kernel<C, true>), dim3(waves), dim3(64), lds, s, lp);
+    else hipLaunchKernelGGL((sim_kernel<C, false>), dim3(waves), dim3(64), lds, s, lp);
     return hipGetLastError();
 }
 
-hipError_t launch_sim(const DevModel &m, const SimParams &p, int slots, bool trace, int waves,
-                      size_t lds_bytes, hipStream_t stream) {
+hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int waves, size_t lds_bytes, hipStream_t stream) {
     switch (slots) {
-    case 1: return launch_c<1>(m, p, trace, waves, lds_bytes, stream);
-    case 2: return launch_c<2>(m, p, trace, waves, lds_bytes, stream);
-    case 4: return launch_c<4>(m, p, trace, waves, lds_bytes, stream);
-    case 8: return launch_c<8>(m, p, trace, waves, lds_bytes, stream);
-    case 16: return launch_c<16>(m, p, trace, waves, lds_bytes, stream);
+    case 1: return launch_c<1>(lp, trace, waves, lds_bytes, stream);
+    case 2: return launch_c<2>(lp, trace, waves, lds_bytes, stream);
+    case 4: return launch_c<4>(lp, trace, waves, lds_bytes, stream);
+    case 8: return launch_c<8>(lp, trace, waves, lds_bytes, stream);
+    case 16: return launch_c<16>(lp, trace, waves, lds_bytes, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -572,12 +628,18 @@ hipError_t sim_occupancy(int slots, bool trace, size_t lds_bytes, int *blocks_pe
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 64, lds_bytes);
 }
 
-hipError_t launch_reduce(const uint8_t *flag, const double *value, const uint32_t *steps,
-                         const uint32_t *trans, uint64_t n, uint32_t max_steps, int blocks,
-                         ReducePartial *out, hipStream_t s) {
-    uint64_t per_block = (n + blocks - 1) / blocks;
-    if (per_block == 0) per_block = 1;
-    hipLaunchKernelGGL(reduce_kernel, dim3(blocks), dim3(256), 0, s, flag, value, steps, trans, n, per_block, max_steps, out);
+hipError_t launch_reduce(const InstDesc *inst, int n_inst, uint32_t blocks_per_inst, ReducePartial *partials,
+                         ReducePartial *acc, int accumulate, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_kernel, dim3(blocks_per_inst, n_inst), dim3(256), 0, s, inst, blocks_per_inst, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(finalize_kernel, dim3((n_inst + 63) / 64), dim3(64), 0, s, partials, n_inst, blocks_per_inst,
+                       acc, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_prep(unsigned long long *counter, uint64_t *round, int advance, hipStream_t s) {
+    hipLaunchKernelGGL(prep_kernel, dim3(1), dim3(64), 0, s, counter, round, advance);
     return hipGetLastError();
 }
 

@@ -136,6 +136,13 @@ class GpuSimulator:
     def __del__(self):
         self.close()
 
+    def autotune(self, query: A.Query, *, seed: int = 1, sample_runs: int = 65536) -> A.Launch:
+        """Pick the fastest launch shape on a sample; later runs use it."""
+        best = A.Launch()
+        A.check(A.lib().smc_autotune(self._h, C.byref(query), C.c_uint64(seed), C.c_uint64(sample_runs),
+                                     C.byref(best)))
+        return best
+
     def run(self, query: A.Query, *, seed: int, n_runs: int, run_begin: int = 0,
             lanes_per_run: int = 0, waves: int = 0, waves_per_cu: int = 0,
             per_run: tuple[str, ...] = ()) -> RunResult:
@@ -153,9 +160,44 @@ class GpuSimulator:
         res = A.Result()
         A.check(A.lib().smc_run(self._h, C.byref(query), C.c_uint64(seed), C.c_uint64(run_begin),
                                 C.c_uint64(n_runs), C.byref(launch), C.byref(outs), C.byref(res)))
-        return RunResult(res.n_runs, res.hits, res.estimate, res.ch_eps, res.clt_half, res.steps,
-                         res.transitions, res.truncated, res.kernel_ms, res.reduce_ms, res.wall_ms, res.lanes_per_run,
-                         res.waves, res.sum, res.sumsq, res.vmin, res.vmax, arrays)
+        return _result(res, arrays)
+
+
+def _result(res: A.Result, per_run=None) -> RunResult:
+    return RunResult(res.n_runs, res.hits, res.estimate, res.ch_eps, res.clt_half, res.steps, res.transitions,
+                     res.truncated, res.kernel_ms, res.reduce_ms, res.wall_ms, res.lanes_per_run, res.waves,
+                     res.sum, res.sumsq, res.vmin, res.vmax, per_run or {})
+
+
+class Batch:
+    """Many (simulator, query, seed, runs) instances in one captured launch
+    per round (smc_batch).  Each round simulates the next n_runs runs of
+    every instance; run() returns per-instance totals over its rounds."""
+
+    def __init__(self, items, *, tune: bool = False):
+        self._h = None
+        self._keep = [it[0] for it in items]
+        arr = (A.BatchItem * len(items))()
+        for k, (sim, query, seed, n_runs, *rest) in enumerate(items):
+            arr[k] = A.BatchItem(sim._h, query, int(seed), int(rest[0]) if rest else 0, int(n_runs))
+        h = C.c_void_p()
+        A.check(A.lib().smc_batch_create(arr, len(items), int(bool(tune)), C.byref(h)))
+        self._h = h
+        self.n = len(items)
+
+    def run(self, rounds: int = 1):
+        res = (A.Result * self.n)()
+        st = A.BatchStats()
+        A.check(A.lib().smc_batch_run(self._h, int(rounds), res, C.byref(st)))
+        return [_result(r) for r in res], st
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and A._lib is not None:
+            A._lib.smc_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def ch_epsilon(n: int, delta: float = 0.05) -> float:

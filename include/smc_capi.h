@@ -118,6 +118,52 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed,
             uint64_t run_begin, uint64_t n_runs, const smc_launch *launch,
             smc_run_outputs *outputs, smc_result *out);
 
+/*
+ * Autotuner: time a sample of `sample_runs` runs of `q` with each candidate
+ * launch shape (lanes per run from next_pow2(components) down to a
+ * sixteenth of it, i.e. 1-16 components per lane, and occupancy caps) and
+ * keep the fastest in the model; smc_run uses it whenever its launch
+ * argument is NULL or all zero.  The chosen shape is written to *best.
+ * Results of smc_run do not depend on the shape (bit-identical per run).
+ */
+int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sample_runs, smc_launch *best);
+
+/* ------------------------------------------------------------ batches */
+
+/*
+ * Many (model, query, run range) instances with uneven component counts in
+ * one launch: instances are grouped by launch shape, each group is one
+ * persistent kernel over a queue of (instance, chunk of runs), the groups
+ * run concurrently, and one reduction folds every instance's runs.  The
+ * whole round is captured once in a HIP graph; each replay simulates the
+ * next n_runs runs of every instance (round r covers run indices
+ * run_begin + r*n_runs ...), and totals accumulate across rounds.
+ * No counterpart in the reference (one model per process, smc.cu:283).
+ */
+typedef struct smc_batch smc_batch;
+
+typedef struct smc_batch_item {
+    smc_model *model;       /* all on one device; must outlive the batch     */
+    smc_query query;
+    uint64_t seed;
+    uint64_t run_begin;
+    uint64_t n_runs;        /* per round                                      */
+} smc_batch_item;
+
+typedef struct smc_batch_stats {
+    double graph_ms;        /* HIP-event time of all replays of the last run  */
+    uint64_t runs;          /* runs simulated by the last smc_batch_run       */
+    uint64_t steps, transitions;
+    int32_t groups;         /* launch shapes (concurrent kernels) per round   */
+    int32_t rounds;
+} smc_batch_stats;
+
+/* tune != 0: autotune one representative instance per group first. */
+int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_batch **out);
+/* Replay `rounds` rounds; results[i] = totals of item i over these rounds. */
+int smc_batch_run(smc_batch *b, int rounds, smc_result *results, smc_batch_stats *stats);
+void smc_batch_destroy(smc_batch *b);
+
 /* Interval helpers (host). */
 double smc_ch_epsilon(uint64_t n, double delta);
 

@@ -1,0 +1,100 @@
+"""Batched multi-instance launches, graph-replayed rounds and the autotuner.
+
+A batch groups instances with uneven component counts by launch shape, runs
+the groups concurrently and reduces every instance on the GPU; each replay of
+its captured graph simulates the next run range of every instance.  Per-run
+results do not depend on the launch, so every per-instance count must equal
+the count of an eager smc_run over the same run range.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cusmc_amd as cs
+from cusmc_amd import generators as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances():
+    specs = []
+    for n in (4, 9, 16, 33):
+        xml, _ = G.fischer_network(n)
+        specs.append((cs.ParsedNetwork.from_string(xml, "P0.cs"), lambda net: cs.reach_query(60.0)))
+    for n, beta in ((16, 0.4), (40, 0.3), (100, 0.25)):
+        xml, _ = G.sir_network(n, beta, 0.1)
+        specs.append((cs.ParsedNetwork.from_string(xml, ""),
+                      lambda net, n=n: cs.var_query(net.variable_id("I"), 30.0, threshold=n // 3)))
+    xml, _ = G.race_network(8)
+    specs.append((cs.ParsedNetwork.from_string(xml, ""),
+                  lambda net: cs.var_query(net.variable_id("count"), 30.0, threshold=14)))
+    return [(net, qf(net)) for net, qf in specs]
+
+
+@pytest.fixture(scope="module")
+def sims():
+    inst = _instances()
+    sims = [(cs.GpuSimulator(net), q) for net, q in inst]
+    yield sims
+    for s, _ in sims:
+        s.close()
+
+
+def test_batch_matches_eager_runs(sims):
+    n = 3000
+    items = [(s, q, 100 + k, n) for k, (s, q) in enumerate(sims)]
+    b = cs.Batch(items)
+    try:
+        res, st = b.run(1)
+    finally:
+        b.close()
+    print(f"[batch] groups={st.groups} runs={st.runs} graph_ms={st.graph_ms:.2f}", flush=True)
+    assert st.runs == n * len(sims) and st.groups >= 3
+    for (s, q), r, k in zip(sims, res, range(len(sims))):
+        e = s.run(q, seed=100 + k, n_runs=n)
+        assert r.hits == e.hits and r.steps == e.steps and r.transitions == e.transitions
+        assert r.sum == pytest.approx(e.sum, rel=1e-12)
+
+
+def test_graph_rounds_interleaved_with_eager_work(sims):
+    """Three replays of the captured round with eager runs and syncs between
+    them: replay r must reproduce the eager result of run range r."""
+    n = 1500
+    items = [(s, q, 7, n) for s, q in sims[:4]]
+    b = cs.Batch(items)
+    try:
+        for rnd in range(3):
+            res, _ = b.run(1)
+            for (s, q), r in zip(sims[:4], res):
+                e = s.run(q, seed=7, n_runs=n, run_begin=rnd * n)
+                assert r.hits == e.hits and r.steps == e.steps, (rnd, r.hits, e.hits)
+        res, st = b.run(2)  # rounds 3 and 4 in one call
+        assert st.rounds == 2
+        for (s, q), r in zip(sims[:4], res):
+            e = s.run(q, seed=7, n_runs=2 * n, run_begin=3 * n)
+            assert r.hits == e.hits and r.transitions == e.transitions
+    finally:
+        b.close()
+
+
+def test_autotune_picks_a_shape_and_keeps_results(sims):
+    s, q = sims[3]  # fischer-33 -> 64 leaves
+    base = s.run(q, seed=5, n_runs=2000, lanes_per_run=64, per_run=("digest",))
+    best = s.autotune(q, seed=1, sample_runs=8192)
+    assert best.lanes_per_run in (64, 32, 16, 8, 4)
+    tuned = s.run(q, seed=5, n_runs=2000, per_run=("digest",))
+    assert tuned.lanes_per_run == best.lanes_per_run
+    assert np.array_equal(base.per_run["digest"], tuned.per_run["digest"])
+
+
+def test_batch_with_autotune(sims):
+    items = [(s, q, 11, 2000) for s, q in sims]
+    b = cs.Batch(items, tune=True)
+    try:
+        res, st = b.run(1)
+    finally:
+        b.close()
+    for (s, q), r in zip(sims, res):
+        e = s.run(q, seed=11, n_runs=2000)
+        assert r.hits == e.hits and r.steps == e.steps
