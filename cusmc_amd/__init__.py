@@ -6,8 +6,8 @@ out) with a from-scratch HIP run kernel behind a C ABI
 (include/smc_capi.h).
 """
 from ._capi import SmcError, lib  # noqa: F401
-from .model import (Batch, GpuSimulator, ParsedNetwork, RunResult, ch_epsilon,  # noqa: F401
-                    reach_query, var_query)
+from .model import (Batch, GpuSimulator, ImportanceSampler, IsResult, ParsedNetwork, RunResult,  # noqa: F401
+                    ch_epsilon, reach_query, var_query)
 
-__all__ = ["Batch", "GpuSimulator", "ParsedNetwork", "RunResult", "SmcError", "ch_epsilon", "lib",
-           "reach_query", "var_query"]
+__all__ = ["Batch", "GpuSimulator", "ImportanceSampler", "IsResult", "ParsedNetwork", "RunResult", "SmcError",
+           "ch_epsilon", "lib", "reach_query", "var_query"]

@@ -83,6 +83,29 @@ class BatchStats(C.Structure):
 
 
 # Every symbol include/smc_capi.h declares, with its signature.
+class IsConfig(C.Structure):
+    _fields_ = [("hidden", C.c_int32), ("rounds", C.c_int32), ("runs_per_round", C.c_uint64),
+                ("elite_frac", C.c_double), ("epochs", C.c_int32), ("learning_rate", C.c_float),
+                ("tilt_max", C.c_float), ("reserved", C.c_int32), ("seed", C.c_uint64)]
+
+
+class IsRound(C.Structure):
+    _fields_ = [("level", C.c_double), ("elites", C.c_uint64), ("hits", C.c_uint64),
+                ("estimate", C.c_double), ("ms", C.c_double)]
+
+
+class IsResult(C.Structure):
+    _fields_ = [("n_runs", C.c_uint64), ("hits", C.c_uint64), ("estimate", C.c_double),
+                ("rel_half_width", C.c_double), ("ms", C.c_double), ("ess", C.c_double),
+                ("steps", C.c_uint64), ("transitions", C.c_uint64)]
+
+
+class IsOutputs(C.Structure):
+    _fields_ = [("flag", C.POINTER(C.c_uint8)), ("log2lr", C.POINTER(C.c_double)),
+                ("digest", C.POINTER(C.c_uint64)), ("steps", C.POINTER(C.c_uint32)),
+                ("transitions", C.POINTER(C.c_uint32)), ("time", C.POINTER(C.c_double))]
+
+
 SIGNATURES = {
     "smc_parse_uppaal": (i32, [C.c_char_p, C.c_char_p, P(C.c_void_p)]),
     "smc_parse_uppaal_string": (i32, [C.c_char_p, C.c_char_p, P(C.c_void_p)]),
@@ -99,6 +122,19 @@ SIGNATURES = {
     "smc_batch_create": (i32, [P(BatchItem), C.c_int, C.c_int, P(C.c_void_p)]),
     "smc_batch_run": (i32, [C.c_void_p, C.c_int, P(Result), P(BatchStats)]),
     "smc_batch_destroy": (None, [C.c_void_p]),
+    "smc_is_create": (i32, [C.c_void_p, P(Query), P(IsConfig), P(C.c_void_p)]),
+    "smc_is_train": (i32, [C.c_void_p, P(IsRound)]),
+    "smc_is_estimate": (i32, [C.c_void_p, u64, u64, u64, u64, f64, P(IsResult)]),
+    "smc_is_sample": (i32, [C.c_void_p, P(C.c_float), C.c_int, u64, u64, u64, C.c_int, C.c_int, P(IsOutputs),
+                            P(IsResult)]),
+    "smc_is_shape": (i32, [C.c_void_p, P(C.c_int32), P(C.c_int32), P(C.c_int32), P(u64)]),
+    "smc_is_get_params": (i32, [C.c_void_p, P(C.c_float)]),
+    "smc_is_set_params": (i32, [C.c_void_p, P(C.c_float)]),
+    "smc_is_mlp_eval": (i32, [C.c_void_p, P(C.c_float), C.c_int, P(C.c_float), P(C.c_float), P(C.c_float)]),
+    "smc_is_record": (i32, [C.c_void_p, P(C.c_float), u64, P(u64), C.c_int, P(C.c_float), C.c_int, P(C.c_uint16),
+                            P(C.c_float), P(C.c_float), u64, P(u64)]),
+    "smc_is_ce_gradient": (i32, [C.c_void_p, P(C.c_float)]),
+    "smc_is_destroy": (None, [C.c_void_p]),
     "smc_ch_epsilon": (f64, [u64, f64]),
     "smc_last_error": (C.c_char_p, []),
     "smc_version": (C.c_char_p, []),

@@ -28,7 +28,7 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", 
              "-Wno-unused-command-line-argument"]
 
 HOST_SOURCES = ["network.cpp", "uppaal.cpp", "compile.cpp"]
-HIP_SOURCES = ["capi.cpp", "sim_kernel.hip"]
+HIP_SOURCES = ["capi.cpp", "is.cpp", "sim_kernel.hip", "is_kernels.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -151,6 +151,7 @@ ModelImage compile_model(const smc_network &n) {
             img.clocks.push_back((uint16_t)v);
         }
     img.comp_init_loc.assign(n.comp_init_loc, n.comp_init_loc + n.n_components);
+    img.comp_first_loc.assign(n.comp_first_loc, n.comp_first_loc + n.n_components + 1);
 
     for (int l = 0; l < n.n_locations; ++l) {
         const smc_location &L = n.locations[l];

@@ -18,6 +18,7 @@ struct ModelImage {
     std::vector<DevInsn> code;
     std::vector<double> var_init;
     std::vector<int32_t> comp_init_loc;
+    std::vector<int32_t> comp_first_loc;
     std::vector<uint16_t> clocks;
     std::vector<int32_t> var_kind;
     int n_comp = 0, n_vars = 0, n_leaves = 1;

@@ -1,0 +1,193 @@
+// device_common.h — device helpers shared by the run kernel and the
+// importance-sampling kernel: expression evaluation, edge enabling and
+// choice, transitions, segment exchanges.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "devmodel.h"
+#include "expr_ops.h"
+#include "smc_stream.h"
+
+namespace {
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int lowest_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+
+// ------------------------------------------------------------ segment exchange
+//
+// Level k of a butterfly over an aligned segment: each lane receives the
+// value held by the sibling block of 2^k lanes.  Levels 0-1 are DPP quad
+// permutes (lane xor 1, xor 2), levels 2-3 DPP mirrors within 8 and 16
+// lanes, level 4 a swizzle xor 16, level 5 a cross-half permute.  The
+// mirrors reach a lane of the sibling block rather than the xor partner,
+// which gives the same value whenever every lane of a block holds the same
+// value — true at every level of a butterfly reduction.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double seg_xchg(double x, int k) {
+    switch (k) {
+    case 0: return dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
+    case 1: return dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
+    case 2: return dpp_f64<0x141>(x);  // row_half_mirror
+    case 3: return dpp_f64<0x140>(x);  // row_mirror
+    case 4: {
+        int lo = __double2loint(x), hi = __double2hiint(x);
+        lo = __builtin_amdgcn_ds_swizzle(lo, 0x401F);  // lane xor 16 within 32
+        hi = __builtin_amdgcn_ds_swizzle(hi, 0x401F);
+        return __hiloint2double(hi, lo);
+    }
+    default: return __shfl_xor(x, 32);
+    }
+}
+
+
+// ------------------------------------------------------------ expressions
+
+// Register-bytecode interpreter (expressions.cu:10-119).  Inlined: an
+// out-of-line call costs the whole kernel registers and time even on models
+// that never reach it.
+__device__ __forceinline__ double eval_code(uint32_t ref, const double *vars, const double *consts,
+                                         const DevInsn *code) {
+    const uint32_t first = ref & 0xFFFFFFu, count = (ref >> 24) & 0x3Fu;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    for (uint32_t k = 0; k < count; ++k) {
+        const DevInsn in = code[first + k];
+        double o[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const uint16_t x = s == 0 ? in.a : s == 1 ? in.b : in.c;
+            const uint32_t kind = x >> 14, i = x & 0x3FFFu;
+            o[s] = kind == SMC_OPND_REG ? (i == 0 ? r0 : i == 1 ? r1 : i == 2 ? r2 : r3)
+                 : kind == SMC_OPND_VAR ? vars[i] : consts[i];
+        }
+        const double v = smc_apply_op(in.op, o[0], o[1], o[2]);
+        r0 = in.dst == 0 ? v : r0;
+        r1 = in.dst == 1 ? v : r1;
+        r2 = in.dst == 2 ? v : r2;
+        r3 = in.dst == 3 ? v : r3;
+    }
+    return r0;
+}
+
+// Value of an expression reference: constants and variables inline.
+__device__ __forceinline__ double eval_ref(uint32_t ref, const double *vars, const DevModel &m) {
+    const uint32_t kind = ref >> 30;
+    if (kind == SMC_REF_CONST) return m.consts[ref & 0x3FFFFFFFu];
+    if (kind == SMC_REF_VAR) return vars[ref & 0x3FFFFFFFu];
+    return eval_code(ref, vars, m.consts, m.code);
+}
+
+// ------------------------------------------------------------ edges
+
+__device__ __forceinline__ bool guard_holds(int op, double x, double r) {
+    switch (op) {
+    case SMC_C_LE: return x <= r;
+    case SMC_C_LT: return x < r;
+    case SMC_C_GE: return x >= r;
+    case SMC_C_GT: return x > r;
+    case SMC_C_EQ: return fabs(x - r) <= SMC_TOL;
+    default: return false;
+    }
+}
+
+// Enabled edges of location `loc` as a bit mask over its edge list:
+// spontaneous edges (recv_chan == 0) or receive edges on channel recv_chan
+// (check_enabled_edges simulation.cu:268-305, SyncInSerial :363-377).
+__device__ __forceinline__ uint64_t enabled_mask(int loc, int recv_chan, const double *vars,
+                                                 const DevModel &m) {
+    const DevLoc &L = m.locs[loc];
+    const int ne = L.edge_n;
+    uint64_t mask = 0;
+    for (int i = 0; i < ne; ++i) {
+        const DevEdge &E = m.edges[L.edge_first + i];
+        if (E.never) continue;
+        if (recv_chan ? (E.channel != -recv_chan) : (E.channel < 0)) continue;
+        bool ok = true;
+        for (int g = 0; g < E.guard_n && ok; ++g) {
+            const DevGuard G = m.guards[E.guard_first + g];
+            ok = guard_holds(G.op, vars[G.var], eval_ref(G.rhs, vars, m));
+        }
+        if (ok) mask |= 1ull << i;
+    }
+    return mask;
+}
+
+__device__ __forceinline__ long long edge_weight(int e, const double *vars, const DevModel &m) {
+    const int w = smc_to_int(eval_ref(m.edges[e].weight, vars, m));
+    return w > 0 ? w : 0;
+}
+
+// Weighted choice among enabled edges (take_transition simulation.cu:43-79
+// and get_sum_edge_weight :9-24, with the weights of the enabled edges).
+// The draw is made only when there is a choice.
+__device__ __forceinline__ int pick_edge(uint64_t mask, int edge_first, uint64_t seed, uint64_t run,
+                                         uint32_t step, uint32_t purpose, uint32_t comp,
+                                         const double *vars, const DevModel &m) {
+    if ((mask & (mask - 1)) == 0) return lowest_lane(mask);
+    const smc_u32x4 w = smc_draw(seed, run, step, purpose, comp);
+    const double u = smc_u01(w.v[0], w.v[1]);
+    long long W = 0;
+    for (uint64_t r = mask; r; r &= r - 1) W += edge_weight(edge_first + lowest_lane(r), vars, m);
+    const long long rnd = (long long)((double)W * u);
+    long long temp = W;
+    for (uint64_t r = mask; r;) {
+        const int i = 63 - __clzll((long long)r);
+        r &= ~(1ull << i);
+        const long long w2 = edge_weight(edge_first + i, vars, m);
+        if (temp - w2 <= rnd) return i;
+        temp -= w2;
+    }
+    return lowest_lane(mask);
+}
+
+struct LaneQuery {
+    int qvar;
+    double qmax, qmin;
+};
+
+// Fire edge `e`: updates in order, query variable tracking, new location
+// (take_transition simulation.cu:88-180).  Returns the destination.
+__device__ __forceinline__ int fire_edge(int e, double *vars, const DevModel &m, LaneQuery &q,
+                                         bool &goal, int &chan) {
+    const DevEdge &E = m.edges[e];
+    for (int k = 0; k < E.upd_n; ++k) {
+        const DevUpd U = m.upds[E.upd_first + k];
+        const double v = eval_ref(U.expr, vars, m);
+        if ((int)U.var == q.qvar) {
+            if (v > q.qmax) q.qmax = v;
+            if (v < q.qmin) q.qmin = v;
+        }
+        vars[U.var] = v;
+    }
+    if (E.dst_goal) goal = true;
+    if (E.channel > 0) chan = E.channel;
+    return E.dst;
+}
+
+template <int C, class T>
+__device__ __forceinline__ T slot_get(const T (&a)[C], int j) {
+    T r = a[0];
+#pragma unroll
+    for (int k = 1; k < C; ++k) r = (j == k) ? a[k] : r;
+    return r;
+}
+template <int C, class T>
+__device__ __forceinline__ void slot_set(T (&a)[C], int j, T v) {
+#pragma unroll
+    for (int k = 0; k < C; ++k) a[k] = (j == k) ? v : a[k];
+}
+
+}  // namespace

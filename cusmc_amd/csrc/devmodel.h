@@ -111,6 +111,7 @@ struct DevModel {
     const double *var_init;
     const int32_t *comp_init_loc;
     const uint16_t *clocks;     // ids of clock variables
+    const int32_t *comp_first_loc;  // [n_comp + 1] location range of each component
     int32_t n_comp;
     int32_t n_vars;
     int32_t n_clocks;

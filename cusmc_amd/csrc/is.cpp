@@ -1,0 +1,821 @@
+// is.cpp — learned importance sampling (smc_is_* in include/smc_capi.h).
+//
+// No counterpart in the reference: cuSMC estimates every probability by plain
+// Monte Carlo (statistics.cuh:126-163, smc.cu:234-257).  Everything here runs
+// on the model's stream:
+//   sampling    lock-step batches of rows; per macro-step the MLP forward pass
+//               (three MFMA GEMMs) turns each row's features into log2 tilts
+//               and is_advance_kernel moves every row to its next transition
+//   training    multilevel cross-entropy: sample a round, keep the runs whose
+//               score reaches the round's level, replay them once under the
+//               tilts that generated them while recording per macro-step the
+//               features and the statistics (A = wins, B = ln2 dt r) per
+//               location, then run epochs of forward, output gradient,
+//               backward GEMMs and Adam over the recorded pairs
+//   estimation  batches of runs under the trained proposal until the 95%
+//               half-width of mean(hit 2^log2lr) is within the target
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "capi_internal.hpp"
+#include "is_kernels.hpp"
+
+using namespace smc::capi;
+
+namespace {
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void ensure(size_t k, const char *what) {
+        if (k <= n) return;
+        release();
+        hip_check(hipMalloc(&p, std::max<size_t>(k, 1) * sizeof(T)), what);
+        n = k;
+    }
+};
+
+size_t rup(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// splitmix64 and Box-Muller: weight initialisation only.
+struct InitRng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return ((double)(next() >> 11) + 0.5) * 0x1.0p-53; }
+    double normal() { return std::sqrt(-2.0 * std::log(uniform())) * std::cos(6.283185307179586 * uniform()); }
+};
+
+uint16_t host_bf16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFF) ? 0x40 : 0));
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+enum class Tilt { NONE, FIXED, MLP };
+
+constexpr uint64_t kTrainRunBase = 1ull << 48;  // run indices of training rounds
+constexpr int kMaxBatchRows = 1 << 18;
+constexpr int kTrainChunk = 1 << 16;
+constexpr double kZ95 = 1.959963984540054;
+
+}  // namespace
+
+struct smc_is {
+    smc_model *m = nullptr;
+    smc_query q{};
+    smc_is_config cfg{};
+    int S = 1, C = 1, R = 64, stride = 1;
+    size_t lds = 0;
+    int n_locs = 0, n_vars = 0, n_comp = 0, F = 0, fpad = 0, H = 128, Lpad = 0;
+    size_t o_w1 = 0, o_b1 = 0, o_w2 = 0, o_b2 = 0, o_w3 = 0, o_b3 = 0, n_params = 0;
+    double qinit = 0.0;
+    std::vector<uint8_t> h_tiltable;
+    // parameters: fp32 master, bf16 copy for the GEMMs, gradient, Adam moments
+    DevBuf<float> w, g, m1, m2;
+    DevBuf<uint16_t> wb;
+    DevBuf<uint8_t> tiltable;
+    DevBuf<float> var_scale, theta_fixed;
+    DevBuf<uint16_t> feat0;
+    // rows of a lock-step batch
+    DevBuf<uint16_t> loc;
+    DevBuf<double> vars, t, log2lr, qmax, qmin;
+    DevBuf<uint32_t> step, trans;
+    DevBuf<int32_t> chan, sender, pos;
+    DevBuf<uint8_t> status;
+    DevBuf<uint64_t> digest, run, runs_in;
+    DevBuf<float> weight;
+    DevBuf<uint16_t> X[2], h1, h2;
+    DevBuf<float> theta;
+    // recorded (run, macro-step) pairs
+    DevBuf<uint16_t> rX;
+    DevBuf<float> rA, rB, rw;
+    size_t rec_cap = 0;
+    // training activations of one chunk of pairs
+    DevBuf<uint16_t> th1, th2, Gb, dh2b, dh1b, xin;
+    DevBuf<float> tth, dh2, dh1, work, xf;
+    DevBuf<unsigned int> counters;  // [0] active rows, [1] record slots used
+    unsigned int *h_counters = nullptr;
+    int adam_t = 0;
+    bool trained = false;
+
+    ~smc_is() {
+        if (m) hipSetDevice(m->device);
+        if (h_counters) hipHostFree(h_counters);
+    }
+
+    hipStream_t stream() const { return m->stream; }
+
+    void ensure_rows(int rows, bool feats) {
+        const size_t r = (size_t)rows;
+        loc.ensure(r * n_comp, "hipMalloc rows");
+        vars.ensure(r * std::max(n_vars, 1), "hipMalloc rows");
+        for (DevBuf<double> *b : {&t, &log2lr, &qmax, &qmin}) b->ensure(r, "hipMalloc rows");
+        step.ensure(r, "hipMalloc rows");
+        trans.ensure(r, "hipMalloc rows");
+        chan.ensure(r, "hipMalloc rows");
+        sender.ensure(r, "hipMalloc rows");
+        pos.ensure(r, "hipMalloc rows");
+        status.ensure(r, "hipMalloc rows");
+        digest.ensure(r, "hipMalloc rows");
+        run.ensure(r, "hipMalloc rows");
+        runs_in.ensure(r, "hipMalloc rows");
+        weight.ensure(r, "hipMalloc rows");
+        if (feats) {
+            X[0].ensure(r * fpad, "hipMalloc features");
+            X[1].ensure(r * fpad, "hipMalloc features");
+            h1.ensure(r * H, "hipMalloc activations");
+            h2.ensure(r * H, "hipMalloc activations");
+            theta.ensure(r * Lpad, "hipMalloc tilts");
+        }
+    }
+
+    void ensure_train(int rows) {
+        const size_t r = (size_t)rows;
+        th1.ensure(r * H, "hipMalloc training");
+        th2.ensure(r * H, "hipMalloc training");
+        dh2b.ensure(r * H, "hipMalloc training");
+        dh1b.ensure(r * H, "hipMalloc training");
+        Gb.ensure(r * Lpad, "hipMalloc training");
+        tth.ensure(r * Lpad, "hipMalloc training");
+        dh2.ensure(r * H, "hipMalloc training");
+        dh1.ensure(r * H, "hipMalloc training");
+        work.ensure(std::max<size_t>((size_t)2048 * 4096, (size_t)1024 * std::max(Lpad, H)), "hipMalloc workspace");
+    }
+
+    IsParams params(int rows, uint64_t seed) const {
+        IsParams P{};
+        P.m = m->dm;
+        P.st = IsRows{loc.p, vars.p, t.p, log2lr.p, qmax.p, qmin.p, step.p, trans.p, chan.p, sender.p,
+                      status.p, digest.p, run.p};
+        P.n_rows = rows;
+        P.lanes = S;
+        P.var_stride = stride;
+        P.lpad = Lpad;
+        P.fpad = fpad;
+        P.seed = seed;
+        P.bound = q.time_bound;
+        P.threshold = q.threshold;
+        P.qinit = qinit;
+        P.reach = q.kind == SMC_QUERY_REACH;
+        P.qvar = q.kind == SMC_QUERY_VAR ? q.var : -1;
+        P.is_max = q.is_max ? 1 : 0;
+        P.max_steps = q.max_steps ? q.max_steps : SMC_DEFAULT_MAX_STEPS;
+        P.tiltable = tiltable.p;
+        P.var_scale = var_scale.p;
+        P.weight = weight.p;
+        P.active_count = counters.p;
+        return P;
+    }
+
+    // theta = MLP(X) for `rows` rows (a multiple of 64); h1, h2 keep the
+    // bf16 hidden activations for the backward pass.
+    void forward(const uint16_t *Xin, int rows, uint16_t *a1, uint16_t *a2, float *out) {
+        GemmArgs g1{};
+        g1.A = Xin; g1.lda = fpad; g1.B = wb.p + o_w1; g1.ldb = H; g1.Cb = a1; g1.ldc = H;
+        g1.M = rows; g1.N = H; g1.K = fpad; g1.bias = w.p + o_b1; g1.relu = 1;
+        hip_check(smc::gemm(g1, false, false, stream()), "MLP layer 1");
+        GemmArgs g2{};
+        g2.A = a1; g2.lda = H; g2.B = wb.p + o_w2; g2.ldb = H; g2.Cb = a2; g2.ldc = H;
+        g2.M = rows; g2.N = H; g2.K = H; g2.bias = w.p + o_b2; g2.relu = 1;
+        hip_check(smc::gemm(g2, false, false, stream()), "MLP layer 2");
+        GemmArgs g3{};
+        g3.A = a2; g3.lda = H; g3.B = wb.p + o_w3; g3.ldb = Lpad; g3.C = out; g3.ldc = Lpad;
+        g3.M = rows; g3.N = Lpad; g3.K = H; g3.bias = w.p + o_b3; g3.clamp_max = cfg.tilt_max;
+        g3.col_mask = tiltable.p;
+        hip_check(smc::gemm(g3, false, false, stream()), "MLP layer 3");
+    }
+
+    static int splits_for(int M, int N, int K) {
+        const int tiles = (M / 64) * (N / 64);
+        int s = std::min(K / 256, 2048 / std::max(tiles, 1));
+        return std::max(1, std::min(s, 256));
+    }
+
+    // Accumulate into g the parameter gradient of sum(G * theta) for rows of
+    // a chunk: tth holds G (fp32), Gb its bf16 copy, th1/th2 the activations.
+    void backward(const uint16_t *Xc, int rows) {
+        hipStream_t s = stream();
+        const int csplit = std::max(1, std::min(1024, rows / 128));
+        GemmArgs a{};
+        a.A = th2.p; a.lda = H; a.B = Gb.p; a.ldb = Lpad; a.C = g.p + o_w3; a.ldc = Lpad;
+        a.M = H; a.N = Lpad; a.K = rows; a.accumulate = 1;
+        hip_check(smc::gemm_splitk(a, true, false, splits_for(H, Lpad, rows), work.p, s), "grad W3");
+        hip_check(smc::colsum(tth.p, rows, Lpad, Lpad, g.p + o_b3, csplit, work.p, s), "grad b3");
+        GemmArgs b{};
+        b.A = Gb.p; b.lda = Lpad; b.B = wb.p + o_w3; b.ldb = Lpad; b.C = dh2.p; b.Cb = dh2b.p; b.ldc = H;
+        b.M = rows; b.N = H; b.K = Lpad; b.mask = th2.p; b.ldm = H;
+        hip_check(smc::gemm(b, false, true, s), "grad h2");
+        GemmArgs c{};
+        c.A = th1.p; c.lda = H; c.B = dh2b.p; c.ldb = H; c.C = g.p + o_w2; c.ldc = H;
+        c.M = H; c.N = H; c.K = rows; c.accumulate = 1;
+        hip_check(smc::gemm_splitk(c, true, false, splits_for(H, H, rows), work.p, s), "grad W2");
+        hip_check(smc::colsum(dh2.p, rows, H, H, g.p + o_b2, csplit, work.p, s), "grad b2");
+        GemmArgs d{};
+        d.A = dh2b.p; d.lda = H; d.B = wb.p + o_w2; d.ldb = H; d.C = dh1.p; d.Cb = dh1b.p; d.ldc = H;
+        d.M = rows; d.N = H; d.K = H; d.mask = th1.p; d.ldm = H;
+        hip_check(smc::gemm(d, false, true, s), "grad h1");
+        GemmArgs e{};
+        e.A = Xc; e.lda = fpad; e.B = dh1b.p; e.ldb = H; e.C = g.p + o_w1; e.ldc = H;
+        e.M = fpad; e.N = H; e.K = rows; e.accumulate = 1;
+        hip_check(smc::gemm_splitk(e, true, false, splits_for(fpad, H, rows), work.p, s), "grad W1");
+        hip_check(smc::colsum(dh1.p, rows, H, H, g.p + o_b1, csplit, work.p, s), "grad b1");
+    }
+
+    struct Batch {
+        Tilt tilt = Tilt::NONE;
+        bool lockstep = true, early_stop = true, record = false;
+        const uint64_t *d_runs = nullptr;  // run of each row, or run_begin + row
+        uint64_t run_begin = 0, seed = 0;
+        int n_valid = 0;
+    };
+
+    // Run the rows of one batch to their end; returns the number of launches.
+    int run_rows(const Batch &b) {
+        hipStream_t s = stream();
+        const int rows = (int)rup((size_t)b.n_valid, 64);
+        const bool feats = b.tilt == Tilt::MLP;
+        if (feats && !b.lockstep) throw std::runtime_error("the MLP proposal needs lock-step advance");
+        ensure_rows(rows, feats);
+        IsParams P = params(rows, b.seed);
+        P.feat = feats ? X[0].p : nullptr;
+        hip_check(smc::is_init_rows(P, b.d_runs, b.run_begin, b.n_valid, feat0.p, s), "init rows");
+        P.early_stop = b.early_stop ? 1 : 0;
+        P.stop_at_transition = b.lockstep ? 1 : 0;
+        if (b.tilt == Tilt::FIXED) { P.theta = theta_fixed.p; P.theta_stride = 0; }
+        if (b.record) {
+            P.rec_X = rX.p; P.rec_A = rA.p; P.rec_B = rB.p; P.rec_w = rw.p;
+            hip_check(hipMemsetAsync(counters.p + 1, 0, sizeof(unsigned int), s), "clear record count");
+        }
+        int cur = 0, launches = 0;
+        for (;;) {
+            if (feats) {
+                forward(X[cur].p, rows, h1.p, h2.p, theta.p);
+                P.theta = theta.p;
+                P.theta_stride = Lpad;
+                P.feat = X[cur ^ 1].p;
+            }
+            if (b.record) {
+                hip_check(smc::is_record_slots(status.p, rows, pos.p, counters.p + 1, (unsigned int)rec_cap, s),
+                          "record slots");
+                P.rec_pos = pos.p;
+            }
+            hip_check(hipMemsetAsync(counters.p, 0, sizeof(unsigned int), s), "clear active count");
+            hip_check(smc::is_launch_advance(P, C, lds, s), "advance kernel launch");
+            ++launches;
+            cur ^= 1;
+            if (!b.lockstep) break;
+            hip_check(hipMemcpyAsync(h_counters, counters.p, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, s),
+                      "copy active count");
+            hip_check(hipStreamSynchronize(s), "advance");
+            if (h_counters[0] == 0) break;
+        }
+        hip_check(hipStreamSynchronize(s), "advance");
+        return launches;
+    }
+
+    size_t rec_pairs = 0;  // recorded pairs (padded to 64 with zero pairs)
+
+    void ensure_records() {
+        if (rec_cap) return;
+        // pairs of (features, wins, ln2 dt r) within 8 GiB
+        const size_t per_pair = (size_t)fpad * 2 + (size_t)Lpad * 8 + 4;
+        rec_cap = std::min<size_t>(((size_t)8 << 30) / per_pair, (size_t)1 << 23) / 64 * 64;
+        rX.ensure(rec_cap * fpad, "hipMalloc records");
+        rA.ensure(rec_cap * Lpad, "hipMalloc records");
+        rB.ensure(rec_cap * Lpad, "hipMalloc records");
+        rw.ensure(rec_cap, "hipMalloc records");
+        ensure_train(kTrainChunk);
+    }
+
+    // Replay runs (device array runs_in, weights in `weight`) under the
+    // batch's proposal and record their macro-steps; returns the pair count.
+    size_t record(Batch b) {
+        ensure_records();
+        b.record = true;
+        b.d_runs = runs_in.p;
+        run_rows(b);
+        hipStream_t st = stream();
+        hip_check(hipMemcpyAsync(h_counters, counters.p, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, st), "copy count");
+        hip_check(hipStreamSynchronize(st), "record");
+        const size_t pairs = std::min<size_t>(h_counters[1], rec_cap);
+        const size_t padded = rup(pairs, 64);
+        if (padded > pairs) {
+            const size_t z = padded - pairs;
+            hip_check(hipMemsetAsync(rX.p + pairs * fpad, 0, z * fpad * 2, st), "clear records");
+            hip_check(hipMemsetAsync(rA.p + pairs * Lpad, 0, z * Lpad * 4, st), "clear records");
+            hip_check(hipMemsetAsync(rB.p + pairs * Lpad, 0, z * Lpad * 4, st), "clear records");
+            hip_check(hipMemsetAsync(rw.p + pairs, 0, z * 4, st), "clear records");
+        }
+        rec_pairs = padded;
+        return pairs;
+    }
+
+    // g = d(-sum_pairs w ln q)/d parameters over the recorded pairs.
+    void ce_gradient() {
+        hipStream_t st = stream();
+        hip_check(hipMemsetAsync(g.p, 0, n_params * 4, st), "clear gradient");
+        for (size_t off = 0; off < rec_pairs; off += kTrainChunk) {
+            const int rows_c = (int)std::min<size_t>(kTrainChunk, rec_pairs - off);
+            const uint16_t *Xc = rX.p + off * fpad;
+            forward(Xc, rows_c, th1.p, th2.p, tth.p);
+            hip_check(smc::is_ce_grad(tth.p, rA.p + off * Lpad, rB.p + off * Lpad, rw.p + off, tiltable.p,
+                                      cfg.tilt_max, rows_c, Lpad, tth.p, Gb.p, st), "output gradient");
+            backward(Xc, rows_c);
+        }
+    }
+
+    template <class T>
+    void fetch(std::vector<T> &dst, const DevBuf<T> &src, size_t n) {
+        dst.resize(n);
+        if (n) hip_check(hipMemcpyAsync(dst.data(), src.p, n * sizeof(T), hipMemcpyDeviceToHost, stream()), "copy rows");
+    }
+
+    void upload_weights(const std::vector<float> &hw) {
+        hip_check(hipMemcpyAsync(w.p, hw.data(), n_params * 4, hipMemcpyHostToDevice, stream()), "upload weights");
+        hip_check(smc::to_bf16(w.p, wb.p, n_params, stream()), "weights to bf16");
+        hip_check(hipStreamSynchronize(stream()), "upload weights");
+    }
+};
+
+extern "C" {
+
+int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in, smc_is **out) {
+    if (!m || !q || !out) return fail(SMC_E_INVALID, "null argument");
+    *out = nullptr;
+    return guarded([&]() -> int {
+        const smc::ModelImage &img = m->img;
+        if (q->kind != SMC_QUERY_REACH && q->kind != SMC_QUERY_VAR) throw std::runtime_error("unknown query kind");
+        if (q->kind == SMC_QUERY_VAR && (q->var < 0 || q->var >= img.n_vars))
+            throw std::runtime_error("query variable id out of range");
+        if (q->kind == SMC_QUERY_VAR && q->is_estimate)
+            throw std::runtime_error("importance sampling estimates probabilities: use a var query with is_estimate = 0");
+        if (!(q->time_bound > 0.0)) throw std::runtime_error("time bound must be positive");
+        auto s = std::make_unique<smc_is>();
+        s->m = m;
+        s->q = *q;
+        smc_is_config c = cfg_in ? *cfg_in : smc_is_config{};
+        if (c.hidden <= 0) c.hidden = 128;
+        if (c.rounds <= 0) c.rounds = 5;
+        if (c.runs_per_round == 0) c.runs_per_round = 100000;
+        if (!(c.elite_frac > 0.0 && c.elite_frac < 1.0)) c.elite_frac = 0.1;
+        if (c.epochs <= 0) c.epochs = 20;
+        if (!(c.learning_rate > 0.f)) c.learning_rate = 3e-3f;
+        if (!(c.tilt_max > 0.f)) c.tilt_max = 6.f;
+        if (c.hidden % 64 || c.hidden > 1024) throw std::runtime_error("hidden width must be a multiple of 64, at most 1024");
+        if (c.runs_per_round > (uint64_t)kMaxBatchRows * 8) throw std::runtime_error("limit: runs_per_round too large");
+        s->cfg = c;
+        hip_check(hipSetDevice(m->device), "hipSetDevice");
+
+        // launch shape of the advance kernel
+        const int P = img.n_leaves;
+        s->S = P < 64 ? P : 64;
+        s->C = P / s->S;
+        if (s->C > 16) throw std::runtime_error("limit: more than 1024 components");
+        s->R = 64 / s->S;
+        s->stride = (img.n_vars > 0 ? img.n_vars : 1) | 1;
+        s->lds = (size_t)s->R * s->stride * sizeof(double);
+        if (s->lds > 60000) throw std::runtime_error("limit: variables exceed the LDS budget");
+
+        s->n_locs = (int)img.locs.size();
+        s->n_vars = img.n_vars;
+        s->n_comp = img.n_comp;
+        s->F = s->n_locs + s->n_vars + 1;
+        s->fpad = (int)rup((size_t)s->F, 64);
+        s->H = c.hidden;
+        s->Lpad = (int)rup((size_t)s->n_locs, 64);
+        const size_t fp = s->fpad, H = s->H, L = s->Lpad;
+        s->o_w1 = 0;
+        s->o_b1 = fp * H;
+        s->o_w2 = s->o_b1 + H;
+        s->o_b2 = s->o_w2 + H * H;
+        s->o_w3 = s->o_b2 + H;
+        s->o_b3 = s->o_w3 + H * L;
+        s->n_params = s->o_b3 + L;
+        s->qinit = (q->kind == SMC_QUERY_VAR && img.var_kind[q->var] == SMC_VAR_INT) ? img.var_init[q->var] : 0.0;
+
+        // tiltable locations: every location that is not zero-delay
+        s->h_tiltable.assign(L, 0);
+        for (int l = 0; l < s->n_locs; ++l) s->h_tiltable[l] = img.locs[l].klass != SMC_K_ZERO;
+        // feature scales: clocks by the time bound, other variables by the
+        // largest initial magnitude or the component count
+        std::vector<float> scale(std::max(s->n_vars, 1), 1.f);
+        double imax = (double)img.n_comp;
+        for (int v = 0; v < img.n_vars; ++v)
+            if (img.var_kind[v] != SMC_VAR_CLOCK) imax = std::max(imax, std::fabs(img.var_init[v]));
+        for (int v = 0; v < img.n_vars; ++v)
+            scale[v] = img.var_kind[v] == SMC_VAR_CLOCK ? (float)(1.0 / q->time_bound) : (float)(1.0 / std::max(1.0, imax));
+        std::vector<uint16_t> f0(fp, 0);
+        for (int cpt = 0; cpt < img.n_comp; ++cpt) f0[img.comp_init_loc[cpt]] = 0x3F80;
+        for (int v = 0; v < img.n_vars; ++v)
+            f0[s->n_locs + v] = host_bf16((float)(img.var_init[v] * (double)scale[v]));
+
+        hipStream_t st = m->stream;
+        s->tiltable.ensure(L, "hipMalloc tiltable");
+        s->var_scale.ensure(scale.size(), "hipMalloc scales");
+        s->theta_fixed.ensure(L, "hipMalloc tilts");
+        s->feat0.ensure(fp, "hipMalloc features");
+        s->counters.ensure(2, "hipMalloc counters");
+        hip_check(hipHostMalloc((void **)&s->h_counters, 2 * sizeof(unsigned int), hipHostMallocDefault), "hipHostMalloc");
+        hip_check(hipMemcpyAsync(s->tiltable.p, s->h_tiltable.data(), L, hipMemcpyHostToDevice, st), "upload");
+        hip_check(hipMemcpyAsync(s->var_scale.p, scale.data(), scale.size() * 4, hipMemcpyHostToDevice, st), "upload");
+        hip_check(hipMemcpyAsync(s->feat0.p, f0.data(), fp * 2, hipMemcpyHostToDevice, st), "upload");
+
+        // parameters: He initialisation of the hidden layers; the output
+        // layer starts at zero, so the untrained proposal is the base law
+        s->w.ensure(s->n_params, "hipMalloc weights");
+        s->wb.ensure(s->n_params, "hipMalloc weights");
+        s->g.ensure(s->n_params, "hipMalloc gradient");
+        s->m1.ensure(s->n_params, "hipMalloc Adam");
+        s->m2.ensure(s->n_params, "hipMalloc Adam");
+        std::vector<float> hw(s->n_params, 0.f);
+        InitRng rng{c.seed ^ 0x5DEECE66Dull};
+        const double s1 = std::sqrt(2.0 / s->F), s2 = std::sqrt(2.0 / (double)H);
+        for (int f = 0; f < s->F; ++f)
+            for (size_t h = 0; h < H; ++h) hw[s->o_w1 + f * H + h] = (float)(rng.normal() * s1);
+        for (size_t i = 0; i < H * H; ++i) hw[s->o_w2 + i] = (float)(rng.normal() * s2);
+        hip_check(hipMemsetAsync(s->m1.p, 0, s->n_params * 4, st), "clear Adam");
+        hip_check(hipMemsetAsync(s->m2.p, 0, s->n_params * 4, st), "clear Adam");
+        s->upload_weights(hw);
+        *out = s.release();
+        return SMC_OK;
+    });
+}
+
+void smc_is_destroy(smc_is *s) { delete s; }
+
+int smc_is_shape(const smc_is *s, int32_t *features_padded, int32_t *hidden, int32_t *locations_padded,
+                 uint64_t *n_params) {
+    if (!s) return fail(SMC_E_INVALID, "null argument");
+    if (features_padded) *features_padded = s->fpad;
+    if (hidden) *hidden = s->H;
+    if (locations_padded) *locations_padded = s->Lpad;
+    if (n_params) *n_params = s->n_params;
+    return SMC_OK;
+}
+
+int smc_is_get_params(const smc_is *s, float *out) {
+    if (!s || !out) return fail(SMC_E_INVALID, "null argument");
+    return guarded([&]() -> int {
+        hip_check(hipSetDevice(s->m->device), "hipSetDevice");
+        hip_check(hipMemcpyAsync(out, s->w.p, s->n_params * 4, hipMemcpyDeviceToHost, s->stream()), "copy weights");
+        hip_check(hipStreamSynchronize(s->stream()), "copy weights");
+        return SMC_OK;
+    });
+}
+
+int smc_is_set_params(smc_is *s, const float *in) {
+    if (!s || !in) return fail(SMC_E_INVALID, "null argument");
+    return guarded([&]() -> int {
+        hip_check(hipSetDevice(s->m->device), "hipSetDevice");
+        s->upload_weights(std::vector<float>(in, in + s->n_params));
+        return SMC_OK;
+    });
+}
+
+int smc_is_mlp_eval(smc_is *s, const float *x, int rows, const float *gout, float *theta_out, float *grad_out) {
+    if (!s || !x || rows <= 0) return fail(SMC_E_INVALID, "null argument or no rows");
+    return guarded([&]() -> int {
+        hip_check(hipSetDevice(s->m->device), "hipSetDevice");
+        hipStream_t st = s->stream();
+        const int rp = (int)rup((size_t)rows, 64);
+        const size_t nx = (size_t)rp * s->fpad, nt = (size_t)rp * s->Lpad;
+        s->ensure_train(rp);
+        s->xf.ensure(std::max(nx, nt), "hipMalloc inputs");
+        s->xin.ensure(nx, "hipMalloc inputs");
+        hip_check(hipMemsetAsync(s->xf.p, 0, nx * 4, st), "clear inputs");
+        hip_check(hipMemcpyAsync(s->xf.p, x, (size_t)rows * s->fpad * 4, hipMemcpyHostToDevice, st), "upload inputs");
+        hip_check(smc::to_bf16(s->xf.p, s->xin.p, nx, st), "inputs to bf16");
+        s->forward(s->xin.p, rp, s->th1.p, s->th2.p, s->tth.p);
+        std::vector<float> th((size_t)rows * s->Lpad);
+        hip_check(hipMemcpyAsync(th.data(), s->tth.p, th.size() * 4, hipMemcpyDeviceToHost, st), "copy tilts");
+        hip_check(hipStreamSynchronize(st), "MLP forward");
+        if (theta_out) memcpy(theta_out, th.data(), th.size() * 4);
+        if (gout) {
+            // the output gradient, zero where the output is clamped or untiltable
+            std::vector<float> gm((size_t)rp * s->Lpad, 0.f);
+            for (int r = 0; r < rows; ++r)
+                for (int l = 0; l < s->Lpad; ++l) {
+                    const size_t i = (size_t)r * s->Lpad + l;
+                    if (s->h_tiltable[l] && std::fabs(th[i]) < s->cfg.tilt_max) gm[i] = gout[i];
+                }
+            hip_check(hipMemcpyAsync(s->tth.p, gm.data(), nt * 4, hipMemcpyHostToDevice, st), "upload output gradient");
+            hip_check(smc::to_bf16(s->tth.p, s->Gb.p, nt, st), "output gradient to bf16");
+            hip_check(hipMemsetAsync(s->g.p, 0, s->n_params * 4, st), "clear gradient");
+            s->backward(s->xin.p, rp);
+            if (grad_out)
+                hip_check(hipMemcpyAsync(grad_out, s->g.p, s->n_params * 4, hipMemcpyDeviceToHost, st), "copy gradient");
+            hip_check(hipStreamSynchronize(st), "MLP backward");
+        }
+        return SMC_OK;
+    });
+}
+
+static void accumulate(const std::vector<uint8_t> &stat, const std::vector<double> &lr, const std::vector<uint32_t> &stp,
+                       const std::vector<uint32_t> &trn, size_t n, double *sw, double *sw2, uint64_t *hits,
+                       uint64_t *steps, uint64_t *transitions) {
+    for (size_t i = 0; i < n; ++i) {
+        if (stat[i] & IS_HIT) {
+            const double w = std::exp2(lr[i]);
+            *sw += w;
+            *sw2 += w * w;
+            ++*hits;
+        }
+        *steps += stp[i];
+        *transitions += trn[i];
+    }
+}
+
+static void fill_is_result(uint64_t n, uint64_t hits, double sw, double sw2, uint64_t steps, uint64_t trans,
+                           double ms, smc_is_result *res) {
+    memset(res, 0, sizeof *res);
+    res->n_runs = n;
+    res->hits = hits;
+    const double N = (double)n;
+    res->estimate = n ? sw / N : 0.0;
+    const double var = n > 1 ? std::max(0.0, (sw2 - sw * sw / N) / (N - 1.0)) : 0.0;
+    res->rel_half_width = res->estimate > 0.0 ? kZ95 * std::sqrt(var / N) / res->estimate : INFINITY;
+    res->ess = sw2 > 0.0 ? sw * sw / sw2 : 0.0;
+    res->steps = steps;
+    res->transitions = trans;
+    res->ms = ms;
+}
+
+int smc_is_sample(smc_is *s, const float *theta, int use_mlp, uint64_t seed, uint64_t run_begin, uint64_t n_runs,
+                  int lockstep, int early_stop, smc_is_outputs *out, smc_is_result *res) {
+    if (!s || !res) return fail(SMC_E_INVALID, "null argument");
+    return guarded([&]() -> int {
+        auto t0 = std::chrono::steady_clock::now();
+        if (n_runs == 0) throw std::runtime_error("n_runs must be positive");
+        hip_check(hipSetDevice(s->m->device), "hipSetDevice");
+        hipStream_t st = s->stream();
+        smc_is::Batch b;
+        b.tilt = theta ? Tilt::FIXED : (use_mlp ? Tilt::MLP : Tilt::NONE);
+        b.lockstep = lockstep != 0;
+        b.early_stop = early_stop != 0;
+        b.seed = seed;
+        if (theta) {
+            std::vector<float> th(s->Lpad, 0.f);
+            memcpy(th.data(), theta, (size_t)s->n_locs * 4);
+            hip_check(hipMemcpyAsync(s->theta_fixed.p, th.data(), th.size() * 4, hipMemcpyHostToDevice, st), "upload tilts");
+        }
+        double sw = 0, sw2 = 0;
+        uint64_t hits = 0, steps = 0, trans = 0;
+        std::vector<uint8_t> stat;
+        std::vector<double> lr, tm;
+        std::vector<uint32_t> stp, trn;
+        std::vector<uint64_t> dig;
+        for (uint64_t done = 0; done < n_runs;) {
+            const int nb = (int)std::min<uint64_t>(n_runs - done, kMaxBatchRows);
+            b.run_begin = run_begin + done;
+            b.n_valid = nb;
+            s->run_rows(b);
+            s->fetch(stat, s->status, nb);
+            s->fetch(lr, s->log2lr, nb);
+            s->fetch(stp, s->step, nb);
+            s->fetch(trn, s->trans, nb);
+            if (out && out->digest) s->fetch(dig, s->digest, nb);
+            if (out && out->time) s->fetch(tm, s->t, nb);
+            hip_check(hipStreamSynchronize(st), "copy rows");
+            accumulate(stat, lr, stp, trn, nb, &sw, &sw2, &hits, &steps, &trans);
+            if (out) {
+                for (int i = 0; i < nb; ++i) {
+                    if (out->flag) out->flag[done + i] = (stat[i] & IS_HIT) ? 1 : 0;
+                    if (out->log2lr) out->log2lr[done + i] = lr[i];
+                    if (out->steps) out->steps[done + i] = stp[i];
+                    if (out->transitions) out->transitions[done + i] = trn[i];
+                    if (out->digest) out->digest[done + i] = dig[i];
+                    if (out->time) out->time[done + i] = tm[i];
+                }
+            }
+            done += (uint64_t)nb;
+        }
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        fill_is_result(n_runs, hits, sw, sw2, steps, trans, ms, res);
+        return SMC_OK;
+    });
+}
+
+int smc_is_train(smc_is *s, smc_is_round *rounds) {
+    if (!s) return fail(SMC_E_INVALID, "null argument");
+    return guarded([&]() -> int {
+        hip_check(hipSetDevice(s->m->device), "hipSetDevice");
+        hipStream_t st = s->stream();
+        const smc_is_config &c = s->cfg;
+        const int N = (int)c.runs_per_round;
+        const bool reach = s->q.kind == SMC_QUERY_REACH;
+        s->ensure_records();
+
+        std::vector<uint8_t> stat;
+        std::vector<double> lr, qm;
+        std::vector<uint32_t> stp, trn;
+        for (int r = 0; r < c.rounds; ++r) {
+            auto t0 = std::chrono::steady_clock::now();
+            // 1. sample a round under the current proposal (each round its own runs)
+            smc_is::Batch b;
+            b.tilt = Tilt::MLP;
+            b.seed = c.seed;
+            b.run_begin = kTrainRunBase + (uint64_t)r * c.runs_per_round;
+            std::vector<uint64_t> run_of;
+            std::vector<double> score;
+            std::vector<uint8_t> hit;
+            std::vector<double> l2;
+            double sw = 0, sw2 = 0;
+            uint64_t hits = 0, steps = 0, trans = 0;
+            for (int done = 0; done < N;) {
+                const int nb = std::min(N - done, kMaxBatchRows);
+                b.n_valid = nb;
+                b.run_begin = kTrainRunBase + (uint64_t)r * c.runs_per_round + (uint64_t)done;
+                s->run_rows(b);
+                s->fetch(stat, s->status, nb);
+                s->fetch(lr, s->log2lr, nb);
+                s->fetch(qm, s->q.is_max ? s->qmax : s->qmin, nb);
+                s->fetch(stp, s->step, nb);
+                s->fetch(trn, s->trans, nb);
+                hip_check(hipStreamSynchronize(st), "copy rows");
+                accumulate(stat, lr, stp, trn, nb, &sw, &sw2, &hits, &steps, &trans);
+                for (int i = 0; i < nb; ++i) {
+                    const bool h = (stat[i] & IS_HIT) != 0;
+                    run_of.push_back(b.run_begin + (uint64_t)i);
+                    hit.push_back(h);
+                    l2.push_back(lr[i]);
+                    // larger is closer to the event
+                    score.push_back(reach ? (h ? 1.0 : 0.0) : (s->q.is_max ? qm[i] : -qm[i]));
+                }
+                done += nb;
+            }
+            // 2. level and elites
+            const double thr = reach ? 0.5 : (s->q.is_max ? s->q.threshold : -s->q.threshold);
+            std::vector<double> sorted = score;
+            const size_t kq = std::min(sorted.size() - 1, (size_t)((1.0 - c.elite_frac) * (double)sorted.size()));
+            std::nth_element(sorted.begin(), sorted.begin() + kq, sorted.end());
+            const double gamma = sorted[kq];
+            std::vector<int> elite;
+            double level;
+            if ((double)hits >= c.elite_frac * N || gamma > thr || reach) {
+                for (int i = 0; i < N; ++i) if (hit[i]) elite.push_back(i);
+                level = s->q.threshold;
+            } else {
+                std::vector<int> ge, gt;
+                for (int i = 0; i < N; ++i) {
+                    if (score[i] >= gamma) ge.push_back(i);
+                    if (score[i] > gamma) gt.push_back(i);
+                }
+                const bool strict = ge.size() > 2 * c.elite_frac * N && gt.size() >= 0.25 * c.elite_frac * N;
+                elite = strict ? gt : ge;
+                level = s->q.is_max ? gamma : -gamma;
+            }
+            smc_is_round info{};
+            info.level = level;
+            info.elites = elite.size();
+            info.hits = hits;
+            info.estimate = sw / (double)N;
+            if (elite.size() > (size_t)kMaxBatchRows) elite.resize(kMaxBatchRows);
+            if (!elite.empty()) {
+                // 3. CE weights: likelihood ratios of the elites, normalised
+                double lmax = -INFINITY;
+                for (int i : elite) lmax = std::max(lmax, l2[i]);
+                std::vector<double> wv(elite.size());
+                double wsum = 0.0;
+                for (size_t k = 0; k < elite.size(); ++k) { wv[k] = std::exp2(l2[elite[k]] - lmax); wsum += wv[k]; }
+                const int ne = (int)elite.size();
+                const int rows = (int)rup((size_t)ne, 64);
+                std::vector<uint64_t> eruns(rows, 0);
+                std::vector<float> ew(rows, 0.f);
+                for (int k = 0; k < ne; ++k) { eruns[k] = run_of[elite[k]]; ew[k] = (float)(wv[k] / wsum); }
+                s->ensure_rows(rows, true);
+                hip_check(hipMemcpyAsync(s->runs_in.p, eruns.data(), rows * 8, hipMemcpyHostToDevice, st), "upload elites");
+                hip_check(hipMemcpyAsync(s->weight.p, ew.data(), rows * 4, hipMemcpyHostToDevice, st), "upload weights");
+                // 4. replay the elites under the same proposal, recording pairs
+                smc_is::Batch rb;
+                rb.tilt = Tilt::MLP;
+                rb.seed = c.seed;
+                rb.n_valid = ne;
+                s->record(rb);
+                // 5. epochs over the recorded pairs
+                for (int e = 0; e < c.epochs && s->rec_pairs > 0; ++e) {
+                    s->ce_gradient();
+                    hip_check(smc::adam_step(s->w.p, s->g.p, s->m1.p, s->m2.p, s->n_params, c.learning_rate,
+                                             ++s->adam_t, s->wb.p, st), "Adam step");
+                }
+                hip_check(hipStreamSynchronize(st), "training");
+            }
+            info.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (rounds) rounds[r] = info;
+        }
+        s->trained = true;
+        return SMC_OK;
+    });
+}
+
+int smc_is_record(smc_is *s, const float *theta, uint64_t seed, const uint64_t *runs, int n_runs, const float *w,
+                  int early_stop, uint16_t *X, float *A, float *B, uint64_t cap, uint64_t *pairs) {
+    if (!s || !runs || n_runs <= 0) return fail(SMC_E_INVALID, "null argument or no runs");
+    return guarded([&]() -> int {
+        hip_check(hipSetDevice(s->m->device), "hipSetDevice");
+        hipStream_t st = s->stream();
+        if (n_runs > kMaxBatchRows) throw std::runtime_error("limit: too many runs to record at once");
+        const int rows = (int)rup((size_t)n_runs, 64);
+        s->ensure_rows(rows, theta == nullptr);
+        std::vector<uint64_t> rr(rows, 0);
+        std::vector<float> ww(rows, 0.f);
+        for (int i = 0; i < n_runs; ++i) { rr[i] = runs[i]; ww[i] = w ? w[i] : 1.f; }
+        hip_check(hipMemcpyAsync(s->runs_in.p, rr.data(), rows * 8, hipMemcpyHostToDevice, st), "upload runs");
+        hip_check(hipMemcpyAsync(s->weight.p, ww.data(), rows * 4, hipMemcpyHostToDevice, st), "upload weights");
+        if (theta) {
+            std::vector<float> th(s->Lpad, 0.f);
+            memcpy(th.data(), theta, (size_t)s->n_locs * 4);
+            hip_check(hipMemcpyAsync(s->theta_fixed.p, th.data(), th.size() * 4, hipMemcpyHostToDevice, st), "upload tilts");
+        }
+        smc_is::Batch b;
+        b.tilt = theta ? Tilt::FIXED : Tilt::MLP;
+        b.seed = seed;
+        b.early_stop = early_stop != 0;
+        b.n_valid = n_runs;
+        const size_t n = s->record(b);
+        if (pairs) *pairs = n;
+        const size_t k = std::min<size_t>(n, cap);
+        if (X) hip_check(hipMemcpyAsync(X, s->rX.p, k * s->fpad * 2, hipMemcpyDeviceToHost, st), "copy records");
+        if (A) hip_check(hipMemcpyAsync(A, s->rA.p, k * s->Lpad * 4, hipMemcpyDeviceToHost, st), "copy records");
+        if (B) hip_check(hipMemcpyAsync(B, s->rB.p, k * s->Lpad * 4, hipMemcpyDeviceToHost, st), "copy records");
+        hip_check(hipStreamSynchronize(st), "copy records");
+        return SMC_OK;
+    });
+}
+
+int smc_is_ce_gradient(smc_is *s, float *grad_out) {
+    if (!s || !grad_out) return fail(SMC_E_INVALID, "null argument");
+    return guarded([&]() -> int {
+        hip_check(hipSetDevice(s->m->device), "hipSetDevice");
+        s->ensure_records();
+        s->ce_gradient();
+        hip_check(hipMemcpyAsync(grad_out, s->g.p, s->n_params * 4, hipMemcpyDeviceToHost, s->stream()), "copy gradient");
+        hip_check(hipStreamSynchronize(s->stream()), "gradient");
+        return SMC_OK;
+    });
+}
+
+int smc_is_estimate(smc_is *s, uint64_t seed, uint64_t run_begin, uint64_t batch_runs, uint64_t max_runs,
+                    double rel_target, smc_is_result *out) {
+    if (!s || !out) return fail(SMC_E_INVALID, "null argument");
+    return guarded([&]() -> int {
+        auto t0 = std::chrono::steady_clock::now();
+        hip_check(hipSetDevice(s->m->device), "hipSetDevice");
+        if (batch_runs == 0) batch_runs = 65536;
+        if (batch_runs > (uint64_t)kMaxBatchRows) batch_runs = kMaxBatchRows;
+        if (max_runs == 0) max_runs = 1ull << 32;
+        if (!(rel_target > 0.0)) rel_target = 0.1;
+        smc_is::Batch b;
+        b.tilt = Tilt::MLP;
+        b.seed = seed;
+        double sw = 0, sw2 = 0;
+        uint64_t hits = 0, steps = 0, trans = 0, done = 0;
+        std::vector<uint8_t> stat;
+        std::vector<double> lr;
+        std::vector<uint32_t> stp, trn;
+        while (done < max_runs) {
+            const int nb = (int)std::min<uint64_t>(batch_runs, max_runs - done);
+            b.run_begin = run_begin + done;
+            b.n_valid = nb;
+            s->run_rows(b);
+            s->fetch(stat, s->status, nb);
+            s->fetch(lr, s->log2lr, nb);
+            s->fetch(stp, s->step, nb);
+            s->fetch(trn, s->trans, nb);
+            hip_check(hipStreamSynchronize(s->stream()), "copy rows");
+            accumulate(stat, lr, stp, trn, nb, &sw, &sw2, &hits, &steps, &trans);
+            done += (uint64_t)nb;
+            smc_is_result tmp;
+            fill_is_result(done, hits, sw, sw2, steps, trans, 0.0, &tmp);
+            if (hits >= 10 && tmp.rel_half_width <= rel_target) break;
+        }
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        fill_is_result(done, hits, sw, sw2, steps, trans, ms, out);
+        return SMC_OK;
+    });
+}
+
+}  // extern "C"

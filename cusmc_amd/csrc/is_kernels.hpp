@@ -1,0 +1,92 @@
+// is_kernels.hpp — launch interface of the importance-sampling kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "devmodel.h"
+
+enum { IS_ACTIVE = 1, IS_HIT = 2, IS_GOAL = 4, IS_MID = 8 };
+
+// State of the runs of a lock-step batch, one row per run (device arrays).
+struct IsRows {
+    uint16_t *loc;        // [rows][n_comp]
+    double *vars;         // [rows][n_vars]
+    double *t, *log2lr, *qmax, *qmin;
+    uint32_t *step, *trans;
+    int32_t *chan, *sender;
+    uint8_t *status;      // IS_* bits
+    uint64_t *digest;
+    uint64_t *run;        // global run index (Philox counter)
+};
+
+struct IsParams {
+    DevModel m;
+    IsRows st;
+    int32_t n_rows;
+    int32_t lanes;
+    int32_t var_stride;
+    int32_t lpad;              // row length of theta and of the recorded statistics (>= n_locs)
+    int32_t fpad;              // row length of features (>= n_locs + n_vars + 1)
+    // query
+    uint64_t seed;
+    double bound, threshold, qinit;
+    int32_t reach, qvar, is_max;
+    uint32_t max_steps;
+    int32_t stop_at_transition; // 1: lock-step macro-steps, 0: run to the end
+    int32_t early_stop;         // end a var-probability run once it is decided
+    // proposal
+    const float *theta;         // log2 tilts per location, or null (no tilt)
+    int32_t theta_stride;       // row stride of theta (0: one row for every run)
+    const uint8_t *tiltable;    // [lpad]
+    uint16_t *feat;             // [rows][fpad] bf16 features of the end state, or null
+    const float *var_scale;     // [n_vars]
+    // recording of cross-entropy statistics (replay of elite runs)
+    const int32_t *rec_pos;     // [rows] slot of this macro-step's record, -1: none; null: no recording
+    uint16_t *rec_X;            // [slots][fpad] features at the macro-step's start
+    float *rec_A;               // [slots][lpad] exponential wins per location
+    float *rec_B;               // [slots][lpad] ln2 * dt * base rate per location
+    float *rec_w;               // [slots] weight of the row
+    const float *weight;        // [rows]
+    unsigned int *active_count; // rows still active after the launch
+};
+
+struct GemmArgs {
+    const uint16_t *A;  int lda;     // bf16
+    const uint16_t *B;  int ldb;     // bf16
+    float *C;           int ldc;     // fp32 out (optional)
+    uint16_t *Cb;                    // bf16 copy of the output (optional, ldc)
+    int M, N, K;
+    int accumulate;                  // C += result
+    const float *bias;               // [N]
+    int relu;
+    const uint16_t *mask; int ldm;   // zero where mask (bf16 bits) is 0
+    float clamp_max;                 // > 0: clamp to [-c, c] ...
+    const uint8_t *col_mask;         // ... and zero columns whose mask is 0
+    int kchunk;                      // K range per blockIdx.z (set by the launchers)
+    size_t zstride;                  // output layer stride per blockIdx.z
+};
+
+namespace smc {
+
+hipError_t is_launch_advance(const IsParams &p, int slots, size_t lds, hipStream_t s);
+// Reset rows [0, rows) to the initial state of runs runs[i] (or run_begin + i);
+// rows >= n_valid are inactive.  feat0 is the feature row of the initial state.
+hipError_t is_init_rows(const IsParams &p, const uint64_t *runs, uint64_t run_begin, int n_valid,
+                        const uint16_t *feat0, hipStream_t s);
+// Record slots for the active rows, in row order, after *base; advances *base.
+hipError_t is_record_slots(const uint8_t *status, int n_rows, int32_t *pos, unsigned int *base, unsigned int cap,
+                           hipStream_t s);
+// Cross-entropy gradient wrt the MLP outputs (see is_kernels.hip).
+hipError_t is_ce_grad(const float *theta, const float *A, const float *B, const float *w, const uint8_t *tiltable,
+                      float tilt_max, int rows, int lpad, float *G, uint16_t *Gb, hipStream_t s);
+hipError_t gemm(const GemmArgs &g, bool ta, bool tb, hipStream_t s);
+// dst (+)= op(A) op(B) with K split over `splits` workgroup layers (partial
+// sums in `work`, splits * M * N floats, folded in a fixed order).
+hipError_t gemm_splitk(const GemmArgs &g, bool ta, bool tb, int splits, float *work, hipStream_t s);
+// dst[N] (+)= column sums of X[rows][ld] (first N columns), split over rows.
+hipError_t colsum(const float *X, int rows, int ld, int N, float *dst, int splits, float *work, hipStream_t s);
+hipError_t to_bf16(const float *x, uint16_t *y, size_t n, hipStream_t s);
+hipError_t adam_step(float *w, const float *g, float *m1, float *m2, size_t n, float lr, int t, uint16_t *wb,
+                     hipStream_t s);
+
+}  // namespace smc

@@ -200,5 +200,140 @@ class Batch:
         self.close()
 
 
+@dataclass
+class IsResult:
+    n_runs: int
+    hits: int
+    estimate: float
+    rel_half_width: float
+    ms: float
+    ess: float
+    steps: int
+    transitions: int
+    per_run: dict = field(default_factory=dict)
+
+    @property
+    def half_width(self) -> float:
+        return self.rel_half_width * self.estimate
+
+
+def _is_result(r: A.IsResult, per_run=None) -> IsResult:
+    return IsResult(r.n_runs, r.hits, r.estimate, r.rel_half_width, r.ms, r.ess, r.steps, r.transitions,
+                    per_run or {})
+
+
+class ImportanceSampler:
+    """Learned importance sampling of one probability query (smc_is): an MLP
+    over the runs' states tilts the exponential rates; every run carries its
+    log2 likelihood ratio so that mean(hit * 2^log2lr) stays unbiased."""
+
+    def __init__(self, sim: GpuSimulator, query: A.Query, *, hidden: int = 128, rounds: int = 5,
+                 runs_per_round: int = 100000, elite_frac: float = 0.1, epochs: int = 20,
+                 learning_rate: float = 3e-3, tilt_max: float = 6.0, seed: int = 1):
+        self._h = None
+        self.sim = sim
+        self.query = query
+        self.config = A.IsConfig(int(hidden), int(rounds), int(runs_per_round), float(elite_frac), int(epochs),
+                                 float(learning_rate), float(tilt_max), 0, int(seed))
+        h = C.c_void_p()
+        A.check(A.lib().smc_is_create(sim._h, C.byref(query), C.byref(self.config), C.byref(h)))
+        self._h = h
+        fp, hd, lp, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_uint64()
+        A.check(A.lib().smc_is_shape(self._h, C.byref(fp), C.byref(hd), C.byref(lp), C.byref(n)))
+        self.features_padded, self.hidden, self.locations_padded, self.n_params = fp.value, hd.value, lp.value, n.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and A._lib is not None:
+            A._lib.smc_is_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def train(self):
+        """Cross-entropy rounds; returns one dict per round."""
+        rounds = (A.IsRound * self.config.rounds)()
+        A.check(A.lib().smc_is_train(self._h, rounds))
+        return [dict(level=r.level, elites=r.elites, hits=r.hits, estimate=r.estimate, ms=r.ms) for r in rounds]
+
+    def estimate(self, *, seed: int, run_begin: int = 0, batch_runs: int = 65536, max_runs: int = 0,
+                 rel_target: float = 0.1) -> IsResult:
+        r = A.IsResult()
+        A.check(A.lib().smc_is_estimate(self._h, C.c_uint64(seed), C.c_uint64(run_begin), C.c_uint64(batch_runs),
+                                        C.c_uint64(max_runs), float(rel_target), C.byref(r)))
+        return _is_result(r)
+
+    def sample(self, *, seed: int, n_runs: int, run_begin: int = 0, theta=None, use_mlp: bool = False,
+               lockstep: bool = True, early_stop: bool = False) -> IsResult:
+        """Runs under a fixed per-location tilt (theta), the MLP, or no tilt;
+        per_run holds flag, log2lr, digest, steps, transitions, time."""
+        spec = {"flag": (np.uint8, A.u8), "log2lr": (np.float64, A.f64), "digest": (np.uint64, A.u64),
+                "steps": (np.uint32, A.u32), "transitions": (np.uint32, A.u32), "time": (np.float64, A.f64)}
+        outs = A.IsOutputs()
+        arrays = {}
+        for name, (dt, ct) in spec.items():
+            a = np.zeros(n_runs, dtype=dt)
+            arrays[name] = a
+            setattr(outs, name, a.ctypes.data_as(C.POINTER(ct)))
+        th = None
+        if theta is not None:
+            th = np.ascontiguousarray(theta, dtype=np.float32)
+        r = A.IsResult()
+        A.check(A.lib().smc_is_sample(self._h, th.ctypes.data_as(C.POINTER(C.c_float)) if th is not None else None,
+                                      int(bool(use_mlp)), C.c_uint64(seed), C.c_uint64(run_begin), C.c_uint64(n_runs),
+                                      int(bool(lockstep)), int(bool(early_stop)), C.byref(outs), C.byref(r)))
+        return _is_result(r, arrays)
+
+    def record(self, runs, *, seed: int, theta=None, weights=None, early_stop: bool = False, cap: int = 1 << 16):
+        """Replay runs and return their recorded macro-steps (X bf16 bits,
+        A, B), numbered macro-step-major, run-minor."""
+        rr = np.ascontiguousarray(runs, dtype=np.uint64)
+        th = None if theta is None else np.ascontiguousarray(theta, dtype=np.float32)
+        ww = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        X = np.zeros((cap, self.features_padded), dtype=np.uint16)
+        Aa = np.zeros((cap, self.locations_padded), dtype=np.float32)
+        Bb = np.zeros((cap, self.locations_padded), dtype=np.float32)
+        n = C.c_uint64()
+        A.check(A.lib().smc_is_record(
+            self._h, th.ctypes.data_as(C.POINTER(C.c_float)) if th is not None else None, C.c_uint64(seed),
+            rr.ctypes.data_as(C.POINTER(A.u64)), len(rr), ww.ctypes.data_as(C.POINTER(C.c_float)) if ww is not None else None,
+            int(bool(early_stop)), X.ctypes.data_as(C.POINTER(C.c_uint16)), Aa.ctypes.data_as(C.POINTER(C.c_float)),
+            Bb.ctypes.data_as(C.POINTER(C.c_float)), C.c_uint64(cap), C.byref(n)))
+        k = min(n.value, cap)
+        return X[:k], Aa[:k], Bb[:k]
+
+    def ce_gradient(self) -> np.ndarray:
+        out = np.zeros(self.n_params, dtype=np.float32)
+        A.check(A.lib().smc_is_ce_gradient(self._h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, dtype=np.float32)
+        A.check(A.lib().smc_is_get_params(self._h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_params(self, params) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == self.n_params
+        A.check(A.lib().smc_is_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def mlp_eval(self, x, g=None):
+        """theta = MLP(x) for x [rows, features_padded]; with g [rows,
+        locations_padded] also the parameter gradient of sum(g * theta)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        rows = x.shape[0]
+        theta = np.zeros((rows, self.locations_padded), dtype=np.float32)
+        grad = None
+        gp = None
+        if g is not None:
+            gg = np.ascontiguousarray(g, dtype=np.float32)
+            grad = np.zeros(self.n_params, dtype=np.float32)
+            gp = gg.ctypes.data_as(C.POINTER(C.c_float))
+        A.check(A.lib().smc_is_mlp_eval(self._h, x.ctypes.data_as(C.POINTER(C.c_float)), rows, gp,
+                                        theta.ctypes.data_as(C.POINTER(C.c_float)),
+                                        grad.ctypes.data_as(C.POINTER(C.c_float)) if grad is not None else None))
+        return theta, grad
+
+
 def ch_epsilon(n: int, delta: float = 0.05) -> float:
     return A.lib().smc_ch_epsilon(int(n), float(delta))

@@ -164,6 +164,98 @@ int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_bat
 int smc_batch_run(smc_batch *b, int rounds, smc_result *results, smc_batch_stats *stats);
 void smc_batch_destroy(smc_batch *b);
 
+/* ------------------------------------------------------------ importance sampling */
+
+/*
+ * Learned importance sampling for a rare-event variable-probability query
+ * Pr[<=T](<> v > threshold) (max queries).  The proposal multiplies the
+ * rate of every exponential component by 2^theta(location, state), theta
+ * from an MLP (features: one-hot locations, scaled variables, time; two
+ * hidden layers, BF16 weights on MFMA with FP32 accumulation), trained by
+ * multilevel cross-entropy on trajectories the simulator generates; every
+ * run carries an FP64 log2 likelihood ratio, so the estimator
+ * mean(hit * 2^log2lr) is unbiased for any proposal.  No counterpart in the
+ * reference, which only estimates by plain Monte Carlo.
+ */
+typedef struct smc_is smc_is;
+
+typedef struct smc_is_config {
+    int32_t hidden;          /* hidden width (default 128)                      */
+    int32_t rounds;          /* cross-entropy rounds (default 5)                */
+    uint64_t runs_per_round; /* trajectories per round (default 100000)         */
+    double elite_frac;       /* fraction kept per round (default 0.1)           */
+    int32_t epochs;          /* gradient steps per round (default 20)           */
+    float learning_rate;     /* Adam step (default 3e-3)                        */
+    float tilt_max;          /* bound on |theta| (log2 units, default 6)        */
+    int32_t reserved;
+    uint64_t seed;           /* weight initialisation and sampling streams      */
+} smc_is_config;
+
+typedef struct smc_is_round {
+    double level;            /* elite level of this round                       */
+    uint64_t elites;
+    uint64_t hits;           /* runs reaching the query threshold               */
+    double estimate;         /* IS estimate from this round's runs              */
+    double ms;               /* sampling + training time                        */
+} smc_is_round;
+
+typedef struct smc_is_result {
+    uint64_t n_runs, hits;
+    double estimate;         /* mean(hit * 2^log2lr)                            */
+    double rel_half_width;   /* 95% normal half-width / estimate                */
+    double ms;               /* wall time of the estimation                     */
+    double ess;              /* effective sample size of the hit weights        */
+    uint64_t steps, transitions;
+} smc_is_result;
+
+typedef struct smc_is_outputs {  /* per-run, each n_runs long, all optional     */
+    uint8_t *flag;
+    double *log2lr;
+    uint64_t *digest;
+    uint32_t *steps;
+    uint32_t *transitions;
+    double *time;
+} smc_is_outputs;
+
+int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg, smc_is **out);
+/* Run the cross-entropy rounds; rounds[] (cfg.rounds entries) may be NULL. */
+int smc_is_train(smc_is *s, smc_is_round *rounds);
+/* Sample batches of batch_runs until rel_half_width <= rel_target or
+ * max_runs runs. */
+int smc_is_estimate(smc_is *s, uint64_t seed, uint64_t run_begin, uint64_t batch_runs, uint64_t max_runs,
+                    double rel_target, smc_is_result *out);
+/* Sample n_runs runs with the trained proposal (theta == NULL and
+ * use_mlp != 0), with no tilt (theta == NULL, use_mlp == 0), or with the
+ * fixed tilt theta[location] (log2 units).  lockstep != 0 advances by
+ * macro-steps; early_stop != 0 ends a run once its outcome is settled. */
+int smc_is_sample(smc_is *s, const float *theta, int use_mlp, uint64_t seed, uint64_t run_begin, uint64_t n_runs,
+                  int lockstep, int early_stop, smc_is_outputs *out, smc_is_result *res);
+/* Parameter count, and flat FP32 parameters [W1 b1 W2 b2 W3 b3] (W row-major
+ * in x out, rows of W1 padded to features_padded, columns of W3 to
+ * locations_padded). */
+int smc_is_shape(const smc_is *s, int32_t *features_padded, int32_t *hidden, int32_t *locations_padded,
+                 uint64_t *n_params);
+int smc_is_get_params(const smc_is *s, float *out);
+int smc_is_set_params(smc_is *s, const float *in);
+/* MLP on `rows` feature rows x (rows x features_padded, FP32, rounded to
+ * BF16): theta (rows x locations_padded); with grad_out != NULL also the
+ * gradient of sum(g * theta_pre_clamp) over the parameters (g is rows x
+ * locations_padded; entries at clamped or untiltable outputs are ignored). */
+int smc_is_mlp_eval(smc_is *s, const float *x, int rows, const float *g, float *theta, float *grad_out);
+/* Replay runs[0..n_runs) (seed) under the fixed tilt theta[location], or the
+ * MLP when theta == NULL, recording every macro-step (a macro-step ends with
+ * the first race whose winner fires an edge) with row weights w (NULL: 1).
+ * Pairs are numbered macro-step-major, run-minor; up to cap are copied out:
+ * X [pairs][features_padded] bf16 bits of the features at the macro-step's
+ * start, A [pairs][locations_padded] exponential wins per location, B the
+ * same shape, ln2 * duration * base rate.  *pairs = number recorded. */
+int smc_is_record(smc_is *s, const float *theta, uint64_t seed, const uint64_t *runs, int n_runs, const float *w,
+                  int early_stop, uint16_t *X, float *A, float *B, uint64_t cap, uint64_t *pairs);
+/* Gradient of the cross-entropy loss -sum_pairs w ln q(theta = MLP(X)) over
+ * the recorded pairs, wrt the flat parameters: what one epoch descends. */
+int smc_is_ce_gradient(smc_is *s, float *grad_out);
+void smc_is_destroy(smc_is *s);
+
 /* Interval helpers (host). */
 double smc_ch_epsilon(uint64_t n, double delta);
 

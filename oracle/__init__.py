@@ -16,6 +16,14 @@ reference file:line each step follows):
   weighted pick (Q10)     pinned: UppaalWeightTest.xml 5:10:1
   ?: update, null step    pinned: condTest.xml closed form
   SIR race + broadcast    pinned: exact lumped CTMC (tests/golden)
+  tilted race, log2 LR    pinned: theta = 0 reduces to the plain race bit for
+                          bit; E[hit 2^log2lr] equals the exact CTMC value
+                          under a tilt; log2 LR of a one-component race has
+                          the closed form -theta + (2^theta - 1) r d
+  macro-step statistics   pinned: their sum over a run reproduces the run's
+                          log2 LR derivative by finite differences of theta
+  MLP (mlp.py)            pinned: central finite differences of the fp64
+                          forward pass
 """
 from __future__ import annotations
 
@@ -43,6 +51,12 @@ class OracleResult(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+class OracleIsOutputs(C.Structure):
+    _fields_ = [("flag", C.POINTER(C.c_uint8)), ("log2lr", C.POINTER(C.c_double)),
+                ("digest", C.POINTER(C.c_uint64)), ("steps", C.POINTER(C.c_uint32)),
+                ("transitions", C.POINTER(C.c_uint32)), ("time", C.POINTER(C.c_double))]
+
+
 _lib = None
 
 
@@ -59,6 +73,13 @@ def lib() -> C.CDLL:
         l.oracle_trace.restype = C.c_long
         l.oracle_trace.argtypes = [C.POINTER(A.Network), C.POINTER(A.Query), C.c_uint64, C.c_uint64, C.c_int,
                                    C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_long]
+        l.oracle_is_run.restype = C.c_int
+        l.oracle_is_run.argtypes = [C.POINTER(A.Network), C.POINTER(A.Query), C.c_uint64, C.c_uint64, C.c_uint64,
+                                    C.POINTER(C.c_float), C.c_int, C.c_int, C.POINTER(OracleIsOutputs)]
+        l.oracle_is_record.restype = C.c_long
+        l.oracle_is_record.argtypes = [C.POINTER(A.Network), C.POINTER(A.Query), C.c_uint64, C.c_uint64,
+                                       C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float), C.c_int, C.c_int,
+                                       C.POINTER(C.c_uint16), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_long]
         l.oracle_tree_pick.restype = C.c_double
         l.oracle_tree_pick.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_double, C.POINTER(C.c_int)]
         l.oracle_last_error.restype = C.c_char_p
@@ -127,3 +148,50 @@ def tree_pick(rates, v: float):
     ch = C.c_int()
     total = lib().oracle_tree_pick(r.ctypes.data_as(C.POINTER(C.c_double)), len(r), float(v), C.byref(ch))
     return total, ch.value
+
+
+def _theta_ptr(theta):
+    if theta is None:
+        return None, None
+    th = np.ascontiguousarray(theta, dtype=np.float32)
+    return th, th.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def is_run(network, query, *, seed: int, n_runs: int, run_begin: int = 0, theta=None, early_stop: bool = False,
+           threads: int = 0):
+    """Tilted runs (theta: log2 tilt per location, or None); returns a dict
+    of per-run arrays flag, log2lr, digest, steps, transitions, time."""
+    th, thp = _theta_ptr(theta)
+    spec = {"flag": (np.uint8, C.c_uint8), "log2lr": (np.float64, C.c_double),
+            "digest": (np.uint64, C.c_uint64), "steps": (np.uint32, C.c_uint32),
+            "transitions": (np.uint32, C.c_uint32), "time": (np.float64, C.c_double)}
+    outs = OracleIsOutputs()
+    arrays = {}
+    for name, (dt, ct) in spec.items():
+        a = np.zeros(n_runs, dtype=dt)
+        arrays[name] = a
+        setattr(outs, name, a.ctypes.data_as(C.POINTER(ct)))
+    rc = lib().oracle_is_run(network.net_ptr, C.byref(query), C.c_uint64(seed), C.c_uint64(run_begin),
+                             C.c_uint64(n_runs), thp, int(bool(early_stop)), int(threads), C.byref(outs))
+    if rc != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return arrays
+
+
+def is_record(network, query, *, seed: int, run: int, theta, var_scale, fpad: int, lpad: int,
+              early_stop: bool = False, cap: int = 4096):
+    """Macro-steps of one tilted run: (X bf16 bits [k][fpad], A [k][lpad], B [k][lpad])."""
+    th, thp = _theta_ptr(theta)
+    vs = np.ascontiguousarray(var_scale, dtype=np.float32)
+    X = np.zeros((cap, fpad), dtype=np.uint16)
+    Aa = np.zeros((cap, lpad), dtype=np.float32)
+    Bb = np.zeros((cap, lpad), dtype=np.float32)
+    n = lib().oracle_is_record(network.net_ptr, C.byref(query), C.c_uint64(seed), C.c_uint64(run), thp,
+                               int(bool(early_stop)), vs.ctypes.data_as(C.POINTER(C.c_float)), int(fpad), int(lpad),
+                               X.ctypes.data_as(C.POINTER(C.c_uint16)), Aa.ctypes.data_as(C.POINTER(C.c_float)),
+                               Bb.ctypes.data_as(C.POINTER(C.c_float)), cap)
+    if n < 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    if n > cap:
+        raise RuntimeError(f"run has {n} macro-steps, more than cap={cap}")
+    return X[:n], Aa[:n], Bb[:n]

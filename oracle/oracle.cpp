@@ -128,10 +128,32 @@ struct RaceTree {
 struct RunOut {
     bool flag;
     double value;
+    double log2lr;
     uint32_t steps, transitions;
     uint64_t digest;
     double time;
 };
+
+// Importance-sampling options of a run: per-location log2 tilts of the
+// exponential rates, early stop of decided var-probability runs, and the
+// recording of macro-steps (features at the start of each, wins and
+// ln2 * dt * base rate per location) used by cross-entropy training.
+struct IsOpts {
+    const float *theta = nullptr;
+    bool early_stop = false;
+    const float *var_scale = nullptr;
+    int fpad = 0, lpad = 0;
+    std::vector<uint16_t> *X = nullptr;
+    std::vector<float> *A = nullptr, *B = nullptr;
+};
+
+uint16_t bf16_rne(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFF) ? 0x40 : 0));
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
 
 class Simulator {
 public:
@@ -147,7 +169,16 @@ public:
     }
 
     // events: optional log of (step, component, edge) per fired edge
-    RunOut run(uint64_t run, std::vector<int32_t> *events = nullptr, std::vector<double> *times = nullptr) {
+    RunOut run(uint64_t run, std::vector<int32_t> *events = nullptr, std::vector<double> *times = nullptr,
+               const IsOpts *is = nullptr) {
+        is_ = is;
+        log2lr_ = 0.0;
+        const bool record = is && is->X;
+        bool new_macro = true, open = false;
+        double t_start = 0.0;
+        std::vector<int> loc0;
+        std::vector<double> r0;
+        std::vector<uint32_t> wins;
         const int N = n_.n_components;
         for (int v = 0; v < n_.n_vars; ++v) vars_[v] = n_.var_init[v];
         for (int c = 0; c < N; ++c) loc_[c] = n_.comp_init_loc[c];
@@ -168,9 +199,29 @@ public:
             if (times) times->push_back(t);
         };
 
+        auto close_macro = [&](double t_end) {
+            if (!open) return;
+            const double LN2 = 0.69314718055994530942;
+            const double dt = (t_end < HUGE_VAL ? t_end : q_.time_bound) - t_start;
+            const size_t base = is->A->size();
+            is->A->resize(base + is->lpad, 0.f);
+            is->B->resize(base + is->lpad, 0.f);
+            for (int c = 0; c < N; ++c) {
+                const int l = loc0[c];
+                if (n_.locations[l].type > SMC_LOC_GOAL) continue;  // not tiltable
+                (*is->A)[base + l] = (float)wins[c];
+                (*is->B)[base + l] = (float)(LN2 * dt * r0[c]);
+            }
+            open = false;
+        };
+
         for (;;) {
             if (!(t < q_.time_bound)) break;
             if (reach && goal) { hit = true; break; }
+            if (is && is->early_stop && !reach && (q_.is_max ? qmax_ > q_.threshold : qmin_ < q_.threshold)) {
+                hit = true;
+                break;
+            }
             if (step >= max_steps_) break;
 
             if (chan > 0) {
@@ -190,10 +241,33 @@ public:
                 sender = -1;
             }
 
+            if (record && new_macro) {
+                close_macro(t);
+                // features: one-hot locations, scaled variables, t / bound
+                const size_t base = is->X->size();
+                is->X->resize(base + is->fpad, 0);
+                uint16_t *f = is->X->data() + base;
+                for (int c = 0; c < N; ++c) f[loc_[c]] = 0x3F80;
+                const int nl = n_.n_locations;
+                for (int v = 0; v < n_.n_vars; ++v)
+                    f[nl + v] = bf16_rne((float)(vars_[v] * (double)is->var_scale[v]));
+                f[nl + n_.n_vars] = bf16_rne((float)(t / q_.time_bound));
+                loc0.assign(loc_.begin(), loc_.end());
+                r0.assign(N, 0.0);
+                wins.assign(N, 0);
+                t_start = t;
+                open = true;
+                base_rates_ = &r0;
+            }
+            new_macro = false;
+
             double d;
             int w;
+            bool exp_won = false;
             if (mode_ == ORACLE_LITERAL) race_literal(run, step, &d, &w);
-            else race_canonical(run, step, &d, &w);
+            else race_canonical(run, step, &d, &w, &exp_won);
+            base_rates_ = nullptr;
+            if (record && exp_won) ++wins[w];
             if (!(d < HUGE_VAL)) { t = HUGE_VAL; break; }
 
             for (int v : clocks_) vars_[v] += d;
@@ -206,10 +280,12 @@ public:
                 fire(w, e, &goal, &nc);
                 log(w, e);
                 if (nc > 0) { chan = nc; sender = w; }
+                new_macro = true;
             }
             t = t + d;
             ++step;
         }
+        if (record) close_macro(t);
 
         RunOut o;
         o.value = q_.is_max ? qmax_ : qmin_;
@@ -221,6 +297,7 @@ public:
         for (int v = 0; v < n_.n_vars; ++v) digest = smc_mix(digest, smc_digest_bits(vars_[v]));
         for (int c = 0; c < N; ++c) digest = smc_mix(digest, (uint64_t)(uint32_t)loc_[c]);
         o.digest = digest;
+        o.log2lr = log2lr_;
         return o;
     }
 
@@ -233,7 +310,11 @@ private:
     std::vector<double> leaf_, vars_;
     std::vector<int> loc_, clocks_;
     std::vector<std::vector<int>> recv_;
-    RaceTree tree_;
+    RaceTree tree_, base_tree_;
+    std::vector<double> tilted_;
+    const IsOpts *is_ = nullptr;
+    std::vector<double> *base_rates_ = nullptr;  // receives the base rates of this race
+    double log2lr_ = 0.0;
     int qvar_ = -1;
     double qmax_ = 0, qmin_ = 0;
 
@@ -328,7 +409,7 @@ private:
         return L.rate >= 0 ? eval(n_, L.rate, vars_.data()) : 1.0;
     }
 
-    void race_canonical(uint64_t run, uint32_t step, double *d_out, int *w_out) {
+    void race_canonical(uint64_t run, uint32_t step, double *d_out, int *w_out, bool *exp_won) {
         const int N = n_.n_components;
         int zero = -1, wu = -1;
         double du = HUGE_VAL;
@@ -347,19 +428,39 @@ private:
                 leaf_[c] = r > 0.0 ? r : 0.0;
             }
         }
+        if (base_rates_) for (int c = 0; c < N; ++c) (*base_rates_)[c] = leaf_[c];
         if (zero >= 0) { *d_out = 0.0; *w_out = zero; return; }
-        tree_.build(leaf_, N);
-        const double R = tree_.t[1];
+        const float *theta = is_ ? is_->theta : nullptr;
+        if (theta) {
+            // tilted proposal: rate r 2^theta[location] for exponential components
+            tilted_.assign(leaf_.begin(), leaf_.end());
+            for (int c = 0; c < N; ++c)
+                if (leaf_[c] > 0.0) tilted_[c] = leaf_[c] * smc_exp2((double)theta[loc_[c]]);
+            tree_.build(tilted_, N);
+            base_tree_.build(leaf_, N);
+        } else {
+            tree_.build(leaf_, N);
+        }
+        const double Rt = tree_.t[1];
+        const double R = theta ? base_tree_.t[1] : Rt;
         double dexp = HUGE_VAL;
         int wexp = 0x7FFFFFFF;
-        if (R > 0.0) {
+        if (Rt > 0.0) {
             smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_RACE, 0);
-            dexp = -smc_log2(smc_u01(w.v[0], w.v[1])) / R;
-            wexp = tree_.descend(smc_u01(w.v[2], w.v[3]) * R);
+            dexp = -smc_log2(smc_u01(w.v[0], w.v[1])) / Rt;
+            wexp = tree_.descend(smc_u01(w.v[2], w.v[3]) * Rt);
         }
         if (wu < 0) wu = 0x7FFFFFFF;
+        bool won = false;
         if (du < dexp || (du == dexp && wu < wexp)) { *d_out = du; *w_out = wu; }
-        else { *d_out = dexp; *w_out = wexp; }
+        else { *d_out = dexp; *w_out = wexp; won = true; }
+        if (exp_won) *exp_won = won;
+        if (theta && *d_out < HUGE_VAL) {
+            // log2 of the likelihood ratio of this race: -theta_w (exponential
+            // winner) + (R' - R) d
+            const double th_w = won ? (double)theta[loc_[*w_out]] : 0.0;
+            log2lr_ = log2lr_ - th_w + (Rt - R) * *d_out;
+        }
     }
 
     // cuSMC's own sampling: each component draws (simulation.cu:513-540),
@@ -446,6 +547,64 @@ int oracle_run(const smc_network *net, const smc_query *q, uint64_t seed, uint64
     res->seconds = secs;
     res->threads = threads;
     return 0;
+}
+
+int oracle_is_run(const smc_network *net, const smc_query *q, uint64_t seed, uint64_t run_begin, uint64_t n_runs,
+                  const float *theta, int early_stop, int threads, oracle_is_outputs *out) {
+    if (!net || !q || !out) { g_err = "null argument"; return -1; }
+    if (q->kind == SMC_QUERY_VAR && (q->var < 0 || q->var >= net->n_vars)) { g_err = "query variable out of range"; return -1; }
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    if ((uint64_t)threads > n_runs) threads = (int)std::max<uint64_t>(1, n_runs);
+    IsOpts opts;
+    opts.theta = theta;
+    opts.early_stop = early_stop != 0;
+    auto work = [&](int k) {
+        Simulator sim(*net, *q, seed, ORACLE_CANONICAL);
+        const uint64_t b = n_runs * k / threads, e = n_runs * (k + 1) / threads;
+        for (uint64_t i = b; i < e; ++i) {
+            RunOut o = sim.run(run_begin + i, nullptr, nullptr, &opts);
+            if (out->flag) out->flag[i] = o.flag;
+            if (out->log2lr) out->log2lr[i] = o.log2lr;
+            if (out->digest) out->digest[i] = o.digest;
+            if (out->steps) out->steps[i] = o.steps;
+            if (out->transitions) out->transitions[i] = o.transitions;
+            if (out->time) out->time[i] = o.time;
+        }
+    };
+    if (threads == 1) work(0);
+    else {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < threads; ++k) pool.emplace_back(work, k);
+        for (auto &th : pool) th.join();
+    }
+    return 0;
+}
+
+long oracle_is_record(const smc_network *net, const smc_query *q, uint64_t seed, uint64_t run, const float *theta,
+                      int early_stop, const float *var_scale, int fpad, int lpad, uint16_t *X, float *A, float *B,
+                      long cap) {
+    if (!net || !q || !var_scale) { g_err = "null argument"; return -1; }
+    if (fpad < net->n_locations + net->n_vars + 1 || lpad < net->n_locations) { g_err = "row length too small"; return -1; }
+    std::vector<uint16_t> xs;
+    std::vector<float> as, bs;
+    IsOpts opts;
+    opts.theta = theta;
+    opts.early_stop = early_stop != 0;
+    opts.var_scale = var_scale;
+    opts.fpad = fpad;
+    opts.lpad = lpad;
+    opts.X = &xs;
+    opts.A = &as;
+    opts.B = &bs;
+    Simulator sim(*net, *q, seed, ORACLE_CANONICAL);
+    sim.run(run, nullptr, nullptr, &opts);
+    const long n = (long)(xs.size() / (size_t)fpad);
+    for (long k = 0; k < n && k < cap; ++k) {
+        if (X) memcpy(X + (size_t)k * fpad, xs.data() + (size_t)k * fpad, (size_t)fpad * 2);
+        if (A) memcpy(A + (size_t)k * lpad, as.data() + (size_t)k * lpad, (size_t)lpad * 4);
+        if (B) memcpy(B + (size_t)k * lpad, bs.data() + (size_t)k * lpad, (size_t)lpad * 4);
+    }
+    return n;
 }
 
 void oracle_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {

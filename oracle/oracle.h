@@ -55,6 +55,31 @@ int oracle_run(const smc_network *net, const smc_query *q, uint64_t seed, uint64
 long oracle_trace(const smc_network *net, const smc_query *q, uint64_t seed, uint64_t run,
                   int mode, int32_t *events, double *times, long cap);
 
+/* Importance sampling (canonical race only): exponential rates multiplied
+ * by 2^theta[location] (theta may be NULL: no tilt), per-run log2 likelihood
+ * ratio of the base law against the tilted one; early_stop ends a
+ * var-probability run once its max (min) has crossed the threshold. */
+typedef struct oracle_is_outputs {
+    uint8_t *flag;
+    double *log2lr;
+    uint64_t *digest;
+    uint32_t *steps;
+    uint32_t *transitions;
+    double *time;
+} oracle_is_outputs;
+
+int oracle_is_run(const smc_network *net, const smc_query *q, uint64_t seed, uint64_t run_begin, uint64_t n_runs,
+                  const float *theta, int early_stop, int threads, oracle_is_outputs *out);
+
+/* Macro-steps of one tilted run (a macro-step ends with the first race whose
+ * winner fires an edge): per macro-step k the bf16 features at its start
+ * X[k][fpad] (one-hot locations, vars * var_scale, t / bound, zero padding)
+ * and per location the exponential wins A[k][lpad] and ln2 * dt * base rate
+ * B[k][lpad].  Returns the number of macro-steps (only cap written) or -1. */
+long oracle_is_record(const smc_network *net, const smc_query *q, uint64_t seed, uint64_t run, const float *theta,
+                      int early_stop, const float *var_scale, int fpad, int lpad, uint16_t *X, float *A, float *B,
+                      long cap);
+
 /* Pairwise-tree sum and proportional descent used by the canonical race,
  * exposed for tests: returns the tree sum of rates[0..n) padded to the next
  * power of two, and writes the leaf chosen for target v * sum. */
