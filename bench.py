@@ -1,14 +1,33 @@
-"""Benchmark: statistical model checking throughput on one MI355X.
+"""Benchmark: statistical model checking on one MI355X.  Prints one JSON line.
 
-Workload: Fischer mutual exclusion with 32 processes,
-query Pr[<=300](<> P0.cs), 1,000,000 runs per step, FP64 clocks and time.
-One step = one smc_run() over all runs (run kernel + on-GPU reduction), the
-model tables already resident in HBM.  Prints one JSON line.
+Default workload: Fischer mutual exclusion with 32 processes, query
+Pr[<=300](<> P0.cs), 1,000,000 runs per step, FP64 clocks and time.  One
+step = one smc_run() over all runs (run kernel + on-GPU reduction); the model
+tables are resident in HBM before the timed region, and the launch shape is
+picked by the autotuner before it.
+
+Other workloads (--workload):
+  sir350   agent SIR, N=350, beta=0.3, gamma=0.1, Pr[<=100](<> I>=105), 1e6 runs
+  race8    8 single-clock race components, Pr[<=100](<> count>=40), 1e4 runs
+  batch    256 SIR + 256 Fischer instances of uneven size, 1e5 runs each,
+           one graph-captured round per step
+  is       rare-event SIR (N=100, beta=0.15, gamma=0.1, one infected),
+           Pr[<=50](<> I>=90): wall time to a +-10% 95% interval with the
+           learned importance sampler (training included) against the plain
+           simulator
+
+roofline: the run kernel is bound by VALU issue (tables sit in L2, per-run
+output is a few bytes), so `achieved` counts algorithmic lane-operations:
+the per-race-step figure of the oracle's cost model for the workload
+(measured over 2e4+ runs, stated below) times the race steps of the launch,
+divided by the run kernel's time from HIP events on its stream; the peak is
+256 CUs x 64 lanes x 2.4 GHz.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -17,49 +36,52 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-VALU_PEAK = 256 * 2.4e9  # wave64 VALU instructions per second: 256 CUs x 2.4 GHz x 1/clk/CU
+LANE_PEAK = 256 * 64 * 2.4e9        # VALU lane-operations per second
+WAVE_PEAK = 256 * 2.4e9             # wave64 VALU instructions per second
+# algorithmic lane-ops per race step (oracle cost model; tests pin them)
+COST_PER_STEP = {"fischer32": 334.7, "sir350": 1424.7, "race8": 881.2}
+SEED = 20261202
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--runs", type=int, default=1_000_000)
-    ap.add_argument("--processes", type=int, default=32)
-    ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(runs shard by run range; see DESIGN.md)", file=sys.stderr)
-        return 2
-
+def _models(workload):
     import cusmc_amd as cs
     from cusmc_amd import generators as G
+    if workload == "fischer32":
+        net = cs.ParsedNetwork.from_string(G.fischer_network(32)[0], "P0.cs")
+        return net, cs.reach_query(300.0), 1_000_000, "fischer-32 Pr[<=300](<> P0.cs)"
+    if workload == "sir350":
+        net = cs.ParsedNetwork.from_string(G.sir_network(350, 0.3, 0.1, seed=SEED)[0], "")
+        return net, cs.var_query(net.variable_id("I"), 100.0, threshold=104), 1_000_000, \
+            "sir-350 Pr[<=100](<> I>=105)"
+    if workload == "race8":
+        net = cs.ParsedNetwork.from_string(G.race_network(8, seed=SEED)[0], "")
+        return net, cs.var_query(net.variable_id("count"), 100.0, threshold=39), 10_000, \
+            "race-8 Pr[<=100](<> count>=40)"
+    raise ValueError(workload)
 
-    xml, _ = G.fischer_network(args.processes)
-    net = cs.ParsedNetwork.from_string(xml, "P0.cs")
-    q = cs.reach_query(300.0)
+
+def bench_plain(args) -> dict:
+    import cusmc_amd as cs
+    net, q, runs, name = _models(args.workload)
+    runs = args.runs or runs
     sim = cs.GpuSimulator(net, device=0)
+    tuned = sim.autotune(q, seed=SEED + 999, sample_runs=min(runs, 65536))
     for w in range(args.warmup):
-        sim.run(q, seed=args.seed + 1000 + w, n_runs=args.runs)
+        sim.run(q, seed=SEED + 1000 + w, n_runs=runs)
     results = []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        results.append(sim.run(q, seed=args.seed + k, n_runs=args.runs))
+        results.append(sim.run(q, seed=SEED + k, n_runs=runs))
     wall = time.perf_counter() - t0
     sim.close()
-
-    runs = sum(r.n_runs for r in results)
+    n = sum(r.n_runs for r in results)
     trans = sum(r.transitions for r in results)
     steps = sum(r.steps for r in results)
     kms = sum(r.kernel_ms for r in results)
-    value = runs / wall
+    achieved = COST_PER_STEP[args.workload] * steps / (kms / 1e3)
     line = {
-        "metric": "runs/sec (Fischer-32, Pr[<=300](<> P0.cs)); transitions/sec reported alongside",
-        "value": value,
+        "metric": "runs/sec; transitions/sec reported alongside",
+        "value": n / wall,
         "unit": "runs/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -69,33 +91,167 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded Fischer-32 network generated as UPPAAL XML)",
-        "config": {"workload": f"fischer-{args.processes} Pr[<=300](<> P0.cs)", "runs_per_step": args.runs,
-                   "lanes_per_run": results[-1].lanes_per_run, "waves": results[-1].waves},
+        "data": "synthetic (seeded network generated as UPPAAL XML)",
+        "config": {"workload": name, "runs_per_step": runs, "lanes_per_run": results[-1].lanes_per_run,
+                   "waves": results[-1].waves, "tuned_lanes_per_run": tuned.lanes_per_run},
         "transitions_per_sec": trans / wall,
         "race_steps_per_sec": steps / wall,
-        "kernel_runs_per_sec": runs / (kms / 1e3),
+        "kernel_runs_per_sec": n / (kms / 1e3),
+        "kernel_ms_per_step": kms / args.steps,
         "estimate": results[-1].estimate,
         "ch_eps": results[-1].ch_eps,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": LANE_PEAK, "unit": "lane-op/s",
+                     "frac": achieved / LANE_PEAK, "traffic": None,
+                     "work": f"{COST_PER_STEP[args.workload]} algorithmic lane-ops per race step x "
+                             f"{steps // args.steps} race steps per launch"},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(net, q, args)
-    print(json.dumps(line))
-    return 0
+    return line
+
+
+def bench_batch(args) -> dict:
+    import cusmc_amd as cs
+    from cusmc_amd import generators as G
+    runs = args.runs or 100_000
+    specs = G.batch_instances(seed=SEED)
+    sims, items = [], []
+    for k, sp in enumerate(specs):
+        net = cs.ParsedNetwork.from_string(sp["xml"], sp["loc_query"])
+        sim = cs.GpuSimulator(net, device=0)
+        if sp["kind"] == "sir":
+            q = cs.var_query(net.variable_id(sp["var"]), sp["time_bound"], threshold=sp["threshold"])
+        else:
+            q = cs.reach_query(sp["time_bound"])
+        sims.append(sim)
+        items.append((sim, q, SEED + k, runs))
+    t_build = time.perf_counter()
+    b = cs.Batch(items, tune=True)
+    t_build = time.perf_counter() - t_build
+    for _ in range(args.warmup):
+        b.run(1)
+    t0 = time.perf_counter()
+    gms = 0.0
+    tot_runs = tot_trans = tot_steps = 0
+    for _ in range(args.steps):
+        res, st = b.run(1)
+        gms += st.graph_ms
+        tot_runs += st.runs
+        tot_trans += st.transitions
+        tot_steps += st.steps
+    wall = time.perf_counter() - t0
+    groups = st.groups
+    b.close()
+    for s in sims:
+        s.close()
+    return {
+        "metric": "runs/sec over a batch of uneven instances; transitions/sec reported alongside",
+        "value": tot_runs / wall, "unit": "runs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (256 SIR + 256 Fischer networks generated from the seed)",
+        "config": {"workload": "batch 256 sir + 256 fischer", "runs_per_instance": runs, "groups": groups,
+                   "tune_and_capture_s": t_build},
+        "transitions_per_sec": tot_trans / wall, "race_steps_per_sec": tot_steps / wall,
+        "graph_ms_per_step": gms / args.steps,
+    }
+
+
+def bench_is(args) -> dict:
+    """Time to +-10% at 95%: learned IS (training included) vs plain."""
+    import cusmc_amd as cs
+    from cusmc_amd import generators as G
+    net = cs.ParsedNetwork.from_string(G.sir_network(100, 0.15, 0.1, initial_infected=1, seed=SEED)[0], "")
+    q = cs.var_query(net.variable_id("I"), 50.0, threshold=89)
+    exact = 6.638e-6  # exact lumped CTMC of this network and query
+    sim = cs.GpuSimulator(net, device=0)
+    sim.autotune(q, seed=SEED + 999, sample_runs=65536)
+    sim.run(q, seed=SEED + 998, n_runs=1 << 20)  # warm
+    # plain: batches of runs until the CLT half-width is within 10%, capped
+    t0 = time.perf_counter()
+    hits = n = 0
+    batch = args.runs or 8_000_000
+    rel = math.inf
+    while time.perf_counter() - t0 < args.plain_cap_s:
+        r = sim.run(q, seed=SEED, n_runs=batch, run_begin=n)
+        hits += r.hits
+        n += r.n_runs
+        p = hits / n
+        rel = 1.959963984540054 * math.sqrt(p * (1 - p) / n) / p if hits else math.inf
+        if hits >= 10 and rel <= 0.1:
+            break
+    plain_s = time.perf_counter() - t0
+    plain_p = hits / n
+    plain_rate = n / plain_s
+    # runs plain needs for +-10% at the exact p, and the time at its rate
+    need = (1.959963984540054 / 0.1) ** 2 * (1 - exact) / exact
+    plain_to_ci_s = plain_s if rel <= 0.1 else need / plain_rate
+    # learned IS: create, train (5 rounds x 1e5), estimate to +-10%
+    t1 = time.perf_counter()
+    s = cs.ImportanceSampler(sim, q, rounds=5, runs_per_round=100_000, epochs=args.epochs, seed=SEED)
+    rounds = s.train()
+    t_train = time.perf_counter() - t1
+    est = s.estimate(seed=SEED + 1, batch_runs=65536, max_runs=1 << 24, rel_target=0.1)
+    is_s = time.perf_counter() - t1
+    s.close()
+    sim.close()
+    half = est.half_width
+    plain_half = 1.959963984540054 * math.sqrt(plain_p * (1 - plain_p) / n) if hits else math.inf
+    return {
+        "metric": "time to +-10% relative half-width at 95%, learned IS vs plain (speed-up)",
+        "value": plain_to_ci_s / is_s, "unit": "x", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": is_s * 1e3, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64 (simulation), bf16/fp32 (MLP)",
+        "data": "synthetic (SIR N=100 network generated from the seed)",
+        "config": {"workload": "is sir-100 Pr[<=50](<> I>=90)", "exact": exact, "rounds": 5,
+                   "runs_per_round": 100_000, "epochs": args.epochs},
+        "is_seconds": is_s, "is_train_seconds": t_train, "is_estimate": est.estimate,
+        "is_rel_half_width": est.rel_half_width, "is_runs": est.n_runs, "is_hits": est.hits, "is_ess": est.ess,
+        "plain_seconds_to_ci": plain_to_ci_s, "plain_measured_seconds": plain_s, "plain_runs": n,
+        "plain_hits": hits, "plain_estimate": plain_p, "plain_rel_half_width": rel,
+        "plain_runs_per_sec": plain_rate, "plain_extrapolated": rel > 0.1,
+        "is_inside_plain_interval": bool(hits) and abs(est.estimate - plain_p) <= plain_half + half,
+        "rounds": rounds,
+    }
 
 
 def cpu_baseline(net, q, args) -> dict:
-    """The oracle (cuSMC's per-component race, LITERAL mode) on the host
+    """The oracle (cuSMC's per-component race, literal mode) on all host
     cores, on a bounded sample of the same workload."""
     import oracle
     threads = os.cpu_count() or 1
     n = 256
-    res, _ = oracle.run(net, q, seed=args.seed, n_runs=n, mode=oracle.LITERAL, threads=threads)
+    res, _ = oracle.run(net, q, seed=SEED, n_runs=n, mode=oracle.LITERAL, threads=threads)
     rate = n / max(res.seconds, 1e-9)
     n = max(threads, int(rate * args.cpu_seconds))
-    res, _ = oracle.run(net, q, seed=args.seed, n_runs=n, mode=oracle.LITERAL, threads=threads)
+    res, _ = oracle.run(net, q, seed=SEED, n_runs=n, mode=oracle.LITERAL, threads=threads)
     return {"value": n / res.seconds, "unit": "runs/s", "cores": res.threads, "kind": "port",
             "sample": f"{n} runs of the same workload, oracle literal mode, {res.seconds:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="fischer32", choices=["fischer32", "sir350", "race8", "batch", "is"])
+    ap.add_argument("--runs", type=int, default=0, help="runs per step (default: the workload's)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--plain-cap-s", type=float, default=120.0, help="is: cap on the plain run's wall time")
+    ap.add_argument("--epochs", type=int, default=20, help="is: epochs per cross-entropy round")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not run here (the run range would be split "
+              f"across GPUs, one process each, with a reduction of a few scalars)", file=sys.stderr)
+        return 2
+    if args.workload == "batch":
+        line = bench_batch(args)
+    elif args.workload == "is":
+        line = bench_is(args)
+    else:
+        line = bench_plain(args)
+    print(json.dumps(line))
+    return 0
 
 
 if __name__ == "__main__":

@@ -142,3 +142,25 @@ def sir_network(n: int = 350, beta: float = 0.3, gamma: float = 0.1, *, initial_
 
 def log_uniform_int(rng: random.Random, lo: int, hi: int) -> int:
     return int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+
+
+def batch_instances(n_sir: int = 256, n_fischer: int = 256, *, seed: int = DEFAULT_SEED) -> list[dict]:
+    """Uneven instances for one batched launch: SIR networks with N
+    log-uniform in [16, 512], beta uniform in [0.1, 0.5], gamma uniform in
+    [0.05, 0.2] (query Pr[<=100](<> I > N/5)), and Fischer networks with N
+    uniform in {4..64} (query Pr[<=300](<> P0.cs)).  Each entry holds the
+    XML, the location query and the query's parameters."""
+    rng = random.Random(seed)
+    out = []
+    for k in range(n_sir):
+        n = log_uniform_int(rng, 16, 512)
+        beta = rng.uniform(0.1, 0.5)
+        gamma = rng.uniform(0.05, 0.2)
+        xml, meta = sir_network(n, beta, gamma, seed=seed + 1 + k)
+        out.append({"kind": "sir", "xml": xml, "loc_query": "", "var": "I", "threshold": n // 5,
+                    "time_bound": 100.0, **meta})
+    for k in range(n_fischer):
+        n = rng.randint(4, 64)
+        xml, _ = fischer_network(n)
+        out.append({"kind": "fischer", "xml": xml, "loc_query": "P0.cs", "time_bound": 300.0, "n": n})
+    return out

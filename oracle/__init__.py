@@ -80,6 +80,9 @@ def lib() -> C.CDLL:
         l.oracle_is_record.argtypes = [C.POINTER(A.Network), C.POINTER(A.Query), C.c_uint64, C.c_uint64,
                                        C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float), C.c_int, C.c_int,
                                        C.POINTER(C.c_uint16), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_long]
+        l.oracle_cost.restype = C.c_int
+        l.oracle_cost.argtypes = [C.POINTER(A.Network), C.POINTER(A.Query), C.c_uint64, C.c_uint64, C.c_uint64,
+                                  C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
         l.oracle_tree_pick.restype = C.c_double
         l.oracle_tree_pick.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_double, C.POINTER(C.c_int)]
         l.oracle_last_error.restype = C.c_char_p
@@ -195,3 +198,14 @@ def is_record(network, query, *, seed: int, run: int, theta, var_scale, fpad: in
     if n > cap:
         raise RuntimeError(f"run has {n} macro-steps, more than cap={cap}")
     return X[:n], Aa[:n], Bb[:n]
+
+
+def cost(network, query, *, seed: int, n_runs: int, run_begin: int = 0, threads: int = 0):
+    """(steps, transitions, algorithmic lane-ops) of the runs, by the cost
+    model of oracle.cpp."""
+    st, tr, ops = C.c_uint64(), C.c_uint64(), C.c_double()
+    rc = lib().oracle_cost(network.net_ptr, C.byref(query), C.c_uint64(seed), C.c_uint64(run_begin),
+                           C.c_uint64(n_runs), int(threads), C.byref(st), C.byref(tr), C.byref(ops))
+    if rc != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return st.value, tr.value, ops.value

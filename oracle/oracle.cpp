@@ -44,6 +44,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <thread>
 #include <vector>
@@ -62,9 +63,30 @@ int to_int_sat(double x) {
     return (int)x;
 }
 
+// Algorithmic cost model (lane-operations of one run's work, counted by
+// the canonical simulation when enabled): the floor a SIMT implementation
+// must issue, independent of how it maps runs to lanes.  Operand fetches are
+// free; the constants below are the operation counts of the published
+// methods the simulator uses.
+thread_local double g_cost = 0.0;
+thread_local bool g_count = false;
+const double COST_OP = 1.0;       // add, mul, compare, select, logic
+const double COST_DIV = 10.0;     // FP64 reciprocal + 2 Newton steps + correction
+const double COST_SQRT = 10.0;
+const double COST_POW = 60.0;     // log2, multiply, exp2
+const double COST_PHILOX = 100.0; // 10 rounds x (2 mul_lo + 2 mul_hi + 4 xor + 2 key adds)
+const double COST_U01 = 4.0;      // 53-bit combine, convert, scale
+const double COST_LOG2 = 30.0;    // frexp + atanh series
+const double COST_EXP2 = 20.0;
+inline void cost(double c) {
+    if (g_count) g_cost += c;
+}
+
 // Tree evaluation with the operator meanings of expressions.cu:10-95.
 double eval(const smc_network &n, int e, const double *vars) {
     const smc_expr_node &x = n.exprs[e];
+    if (g_count && x.op != SMC_E_LIT && x.op != SMC_E_VAR)
+        g_cost += x.op == SMC_E_DIV ? COST_DIV : x.op == SMC_E_SQRT ? COST_SQRT : x.op == SMC_E_POW ? COST_POW : COST_OP;
     switch (x.op) {
     case SMC_E_LIT: return x.value;
     case SMC_E_VAR: return vars[x.var];
@@ -271,6 +293,7 @@ public:
             if (!(d < HUGE_VAL)) { t = HUGE_VAL; break; }
 
             for (int v : clocks_) vars_[v] += d;
+            cost(COST_OP * (double)clocks_.size() + 2 * COST_OP);  // clocks, time
             std::vector<int> en;
             enabled(loc_[w], 0, &en);
             if (!en.empty()) {
@@ -344,7 +367,9 @@ private:
         for (int k = 0; k < L.edge_count; ++k) {
             const int e = L.edge_first + k;
             const smc_edge &E = n_.edges[e];
+            cost(2 * COST_OP);
             if (recv_chan ? (E.channel != -recv_chan) : (E.channel < 0)) continue;
+            cost(E.guard_count * 2 * COST_OP);
             if (guards_hold(E)) out->push_back(e);
         }
     }
@@ -358,6 +383,7 @@ private:
 
     int pick(const std::vector<int> &en, double u) const {
         if (en.size() == 1) return en[0];
+        cost(COST_PHILOX + COST_U01 + 4 * COST_OP * (double)en.size());
         long long W = 0;
         for (int e : en) W += weight(e);
         const long long rnd = (long long)((double)W * u);
@@ -372,6 +398,7 @@ private:
 
     void fire(int c, int e, bool *goal, int *chan) {
         const smc_edge &E = n_.edges[e];
+        cost(2 * COST_OP + 2 * COST_OP * E.upd_count);
         for (int k = 0; k < E.upd_count; ++k) {
             const smc_update &U = n_.updates[E.upd_first + k];
             const double v = eval(n_, U.expr, vars_.data());
@@ -396,6 +423,7 @@ private:
             const smc_constraint &I = n_.constraints[L.inv_first + k];
             if (!I.uses_variable || n_.var_kind[I.var] != SMC_VAR_CLOCK) continue;
             if (I.op != SMC_C_LE && I.op != SMC_C_LT) continue;
+            cost(4 * COST_OP);
             double ttb = eval(n_, I.rhs, vars_.data()) - vars_[I.var];
             if (I.op == SMC_C_LT) ttb -= 1e-6;
             if (ttb >= 0.0) { m = ttb < m ? ttb : m; bounded = true; }
@@ -413,6 +441,7 @@ private:
         const int N = n_.n_components;
         int zero = -1, wu = -1;
         double du = HUGE_VAL;
+        cost(2 * COST_OP * N);  // delay class and leaf of every component
         for (int c = 0; c < N; ++c) {
             leaf_[c] = 0.0;
             const int type = n_.locations[loc_[c]].type;
@@ -420,6 +449,7 @@ private:
             double M;
             if (bound_of(c, &M)) {
                 if (M == 0.0) { if (zero < 0) zero = c; continue; }
+                cost(COST_PHILOX + COST_U01 + 3 * COST_OP);
                 smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_UNIFORM, (uint32_t)c);
                 const double d = M * smc_u01(w.v[0], w.v[1]);
                 if (d < du) { du = d; wu = c; }
@@ -430,6 +460,13 @@ private:
         }
         if (base_rates_) for (int c = 0; c < N; ++c) (*base_rates_)[c] = leaf_[c];
         if (zero >= 0) { *d_out = 0.0; *w_out = zero; return; }
+        {
+            int P = 1, lv = 0;
+            while (P < N) { P <<= 1; ++lv; }
+            // tree sum, descent, exponential draw, winner against the uniforms
+            cost(COST_OP * (P - 1) + 3 * COST_OP * lv + COST_PHILOX + 2 * COST_U01 + COST_LOG2 + COST_DIV +
+                 4 * COST_OP);
+        }
         const float *theta = is_ ? is_->theta : nullptr;
         if (theta) {
             // tilted proposal: rate r 2^theta[location] for exponential components
@@ -605,6 +642,38 @@ long oracle_is_record(const smc_network *net, const smc_query *q, uint64_t seed,
         if (B) memcpy(B + (size_t)k * lpad, bs.data() + (size_t)k * lpad, (size_t)lpad * 4);
     }
     return n;
+}
+
+int oracle_cost(const smc_network *net, const smc_query *q, uint64_t seed, uint64_t run_begin, uint64_t n_runs,
+                int threads, uint64_t *steps, uint64_t *transitions, double *lane_ops) {
+    if (!net || !q) { g_err = "null argument"; return -1; }
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    if ((uint64_t)threads > n_runs) threads = (int)std::max<uint64_t>(1, n_runs);
+    std::vector<uint64_t> st(threads, 0), tr(threads, 0);
+    std::vector<double> co(threads, 0.0);
+    auto work = [&](int k) {
+        Simulator sim(*net, *q, seed, ORACLE_CANONICAL);
+        g_count = true;
+        g_cost = 0.0;
+        const uint64_t b = n_runs * k / threads, e = n_runs * (k + 1) / threads;
+        for (uint64_t i = b; i < e; ++i) {
+            RunOut o = sim.run(run_begin + i);
+            st[k] += o.steps;
+            tr[k] += o.transitions;
+        }
+        co[k] = g_cost;
+        g_count = false;
+    };
+    if (threads == 1) work(0);
+    else {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < threads; ++k) pool.emplace_back(work, k);
+        for (auto &th : pool) th.join();
+    }
+    if (steps) *steps = std::accumulate(st.begin(), st.end(), (uint64_t)0);
+    if (transitions) *transitions = std::accumulate(tr.begin(), tr.end(), (uint64_t)0);
+    if (lane_ops) *lane_ops = std::accumulate(co.begin(), co.end(), 0.0);
+    return 0;
 }
 
 void oracle_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {

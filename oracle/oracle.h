@@ -80,6 +80,13 @@ long oracle_is_record(const smc_network *net, const smc_query *q, uint64_t seed,
                       int early_stop, const float *var_scale, int fpad, int lpad, uint16_t *X, float *A, float *B,
                       long cap);
 
+/* Algorithmic work of runs [run_begin, run_begin + n_runs) (canonical
+ * race): lane-operations by the cost model in oracle.cpp (one per FP64
+ * add/mul/compare; division, Philox, log2 and the other published methods at
+ * their operation counts), with the race steps and transitions. */
+int oracle_cost(const smc_network *net, const smc_query *q, uint64_t seed, uint64_t run_begin, uint64_t n_runs,
+                int threads, uint64_t *steps, uint64_t *transitions, double *lane_ops);
+
 /* Pairwise-tree sum and proportional descent used by the canonical race,
  * exposed for tests: returns the tree sum of rates[0..n) padded to the next
  * power of two, and writes the leaf chosen for target v * sum. */
