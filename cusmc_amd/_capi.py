@@ -86,12 +86,12 @@ class BatchStats(C.Structure):
 class IsConfig(C.Structure):
     _fields_ = [("hidden", C.c_int32), ("rounds", C.c_int32), ("runs_per_round", C.c_uint64),
                 ("elite_frac", C.c_double), ("epochs", C.c_int32), ("learning_rate", C.c_float),
-                ("tilt_max", C.c_float), ("reserved", C.c_int32), ("seed", C.c_uint64)]
+                ("tilt_max", C.c_float), ("per_location", C.c_int32), ("seed", C.c_uint64)]
 
 
 class IsRound(C.Structure):
     _fields_ = [("level", C.c_double), ("elites", C.c_uint64), ("hits", C.c_uint64),
-                ("estimate", C.c_double), ("ms", C.c_double)]
+                ("estimate", C.c_double), ("ms", C.c_double), ("ess", C.c_double), ("pairs", C.c_uint64)]
 
 
 class IsResult(C.Structure):
@@ -127,7 +127,8 @@ SIGNATURES = {
     "smc_is_estimate": (i32, [C.c_void_p, u64, u64, u64, u64, f64, P(IsResult)]),
     "smc_is_sample": (i32, [C.c_void_p, P(C.c_float), C.c_int, u64, u64, u64, C.c_int, C.c_int, P(IsOutputs),
                             P(IsResult)]),
-    "smc_is_shape": (i32, [C.c_void_p, P(C.c_int32), P(C.c_int32), P(C.c_int32), P(u64)]),
+    "smc_is_shape": (i32, [C.c_void_p, P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_int32), P(u64)]),
+    "smc_is_groups": (C.c_int, [C.c_void_p, P(C.c_int32)]),
     "smc_is_get_params": (i32, [C.c_void_p, P(C.c_float)]),
     "smc_is_set_params": (i32, [C.c_void_p, P(C.c_float)]),
     "smc_is_mlp_eval": (i32, [C.c_void_p, P(C.c_float), C.c_int, P(C.c_float), P(C.c_float), P(C.c_float)]),

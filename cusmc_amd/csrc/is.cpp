@@ -20,6 +20,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <vector>
@@ -91,13 +92,16 @@ struct smc_is {
     int S = 1, C = 1, R = 64, stride = 1;
     size_t lds = 0;
     int n_locs = 0, n_vars = 0, n_comp = 0, F = 0, fpad = 0, H = 128, Lpad = 0;
+    int n_groups = 0, Gpad = 0;  // tilt groups: the MLP's outputs
     size_t o_w1 = 0, o_b1 = 0, o_w2 = 0, o_b2 = 0, o_w3 = 0, o_b3 = 0, n_params = 0;
     double qinit = 0.0;
     std::vector<uint8_t> h_tiltable;
     // parameters: fp32 master, bf16 copy for the GEMMs, gradient, Adam moments
     DevBuf<float> w, g, m1, m2;
     DevBuf<uint16_t> wb;
-    DevBuf<uint8_t> tiltable;
+    DevBuf<uint8_t> tiltable, gtiltable;
+    DevBuf<int32_t> group, member_first, members;
+    std::vector<int32_t> h_group;
     DevBuf<float> var_scale, theta_fixed;
     DevBuf<uint16_t> feat0;
     // rows of a lock-step batch
@@ -109,14 +113,14 @@ struct smc_is {
     DevBuf<uint64_t> digest, run, runs_in;
     DevBuf<float> weight;
     DevBuf<uint16_t> X[2], h1, h2;
-    DevBuf<float> theta;
+    DevBuf<float> theta, thg;
     // recorded (run, macro-step) pairs
     DevBuf<uint16_t> rX;
     DevBuf<float> rA, rB, rw;
     size_t rec_cap = 0;
     // training activations of one chunk of pairs
     DevBuf<uint16_t> th1, th2, Gb, dh2b, dh1b, xin;
-    DevBuf<float> tth, dh2, dh1, work, xf;
+    DevBuf<float> tth, tthg, dh2, dh1, work, xf;
     DevBuf<unsigned int> counters;  // [0] active rows, [1] record slots used
     unsigned int *h_counters = nullptr;
     int adam_t = 0;
@@ -150,6 +154,7 @@ struct smc_is {
             h1.ensure(r * H, "hipMalloc activations");
             h2.ensure(r * H, "hipMalloc activations");
             theta.ensure(r * Lpad, "hipMalloc tilts");
+            thg.ensure(r * Gpad, "hipMalloc tilts");
         }
     }
 
@@ -159,11 +164,12 @@ struct smc_is {
         th2.ensure(r * H, "hipMalloc training");
         dh2b.ensure(r * H, "hipMalloc training");
         dh1b.ensure(r * H, "hipMalloc training");
-        Gb.ensure(r * Lpad, "hipMalloc training");
+        Gb.ensure(r * Gpad, "hipMalloc training");
         tth.ensure(r * Lpad, "hipMalloc training");
+        tthg.ensure(r * Gpad, "hipMalloc training");
         dh2.ensure(r * H, "hipMalloc training");
         dh1.ensure(r * H, "hipMalloc training");
-        work.ensure(std::max<size_t>((size_t)2048 * 4096, (size_t)1024 * std::max(Lpad, H)), "hipMalloc workspace");
+        work.ensure(std::max<size_t>((size_t)2048 * 4096, (size_t)1024 * std::max(Gpad, H)), "hipMalloc workspace");
     }
 
     IsParams params(int rows, uint64_t seed) const {
@@ -191,9 +197,10 @@ struct smc_is {
         return P;
     }
 
-    // theta = MLP(X) for `rows` rows (a multiple of 64); h1, h2 keep the
-    // bf16 hidden activations for the backward pass.
-    void forward(const uint16_t *Xin, int rows, uint16_t *a1, uint16_t *a2, float *out) {
+    // theta = MLP(X) for `rows` rows (a multiple of 64): the group outputs in
+    // og (rows x Gpad), expanded to every location in out (rows x Lpad);
+    // a1, a2 keep the bf16 hidden activations for the backward pass.
+    void forward(const uint16_t *Xin, int rows, uint16_t *a1, uint16_t *a2, float *og, float *out) {
         GemmArgs g1{};
         g1.A = Xin; g1.lda = fpad; g1.B = wb.p + o_w1; g1.ldb = H; g1.Cb = a1; g1.ldc = H;
         g1.M = rows; g1.N = H; g1.K = fpad; g1.bias = w.p + o_b1; g1.relu = 1;
@@ -203,10 +210,11 @@ struct smc_is {
         g2.M = rows; g2.N = H; g2.K = H; g2.bias = w.p + o_b2; g2.relu = 1;
         hip_check(smc::gemm(g2, false, false, stream()), "MLP layer 2");
         GemmArgs g3{};
-        g3.A = a2; g3.lda = H; g3.B = wb.p + o_w3; g3.ldb = Lpad; g3.C = out; g3.ldc = Lpad;
-        g3.M = rows; g3.N = Lpad; g3.K = H; g3.bias = w.p + o_b3; g3.clamp_max = cfg.tilt_max;
-        g3.col_mask = tiltable.p;
+        g3.A = a2; g3.lda = H; g3.B = wb.p + o_w3; g3.ldb = Gpad; g3.C = og; g3.ldc = Gpad;
+        g3.M = rows; g3.N = Gpad; g3.K = H; g3.bias = w.p + o_b3; g3.clamp_max = cfg.tilt_max;
+        g3.col_mask = gtiltable.p;
         hip_check(smc::gemm(g3, false, false, stream()), "MLP layer 3");
+        hip_check(smc::expand_groups(og, Gpad, group.p, n_locs, Lpad, rows, out, stream()), "expand tilts");
     }
 
     static int splits_for(int M, int N, int K) {
@@ -216,18 +224,22 @@ struct smc_is {
     }
 
     // Accumulate into g the parameter gradient of sum(G * theta) for rows of
-    // a chunk: tth holds G (fp32), Gb its bf16 copy, th1/th2 the activations.
+    // a chunk: tth holds G per location (fp32), th1/th2 the activations.  The
+    // gradient of each group output is the sum over its locations (tthg and
+    // its bf16 copy Gb).
     void backward(const uint16_t *Xc, int rows) {
         hipStream_t s = stream();
+        hip_check(smc::group_sum(tth.p, Lpad, member_first.p, members.p, n_groups, Gpad, rows, tthg.p, Gb.p, s),
+                  "group gradient");
         const int csplit = std::max(1, std::min(1024, rows / 128));
         GemmArgs a{};
-        a.A = th2.p; a.lda = H; a.B = Gb.p; a.ldb = Lpad; a.C = g.p + o_w3; a.ldc = Lpad;
-        a.M = H; a.N = Lpad; a.K = rows; a.accumulate = 1;
-        hip_check(smc::gemm_splitk(a, true, false, splits_for(H, Lpad, rows), work.p, s), "grad W3");
-        hip_check(smc::colsum(tth.p, rows, Lpad, Lpad, g.p + o_b3, csplit, work.p, s), "grad b3");
+        a.A = th2.p; a.lda = H; a.B = Gb.p; a.ldb = Gpad; a.C = g.p + o_w3; a.ldc = Gpad;
+        a.M = H; a.N = Gpad; a.K = rows; a.accumulate = 1;
+        hip_check(smc::gemm_splitk(a, true, false, splits_for(H, Gpad, rows), work.p, s), "grad W3");
+        hip_check(smc::colsum(tthg.p, rows, Gpad, Gpad, g.p + o_b3, csplit, work.p, s), "grad b3");
         GemmArgs b{};
-        b.A = Gb.p; b.lda = Lpad; b.B = wb.p + o_w3; b.ldb = Lpad; b.C = dh2.p; b.Cb = dh2b.p; b.ldc = H;
-        b.M = rows; b.N = H; b.K = Lpad; b.mask = th2.p; b.ldm = H;
+        b.A = Gb.p; b.lda = Gpad; b.B = wb.p + o_w3; b.ldb = Gpad; b.C = dh2.p; b.Cb = dh2b.p; b.ldc = H;
+        b.M = rows; b.N = H; b.K = Gpad; b.mask = th2.p; b.ldm = H;
         hip_check(smc::gemm(b, false, true, s), "grad h2");
         GemmArgs c{};
         c.A = th1.p; c.lda = H; c.B = dh2b.p; c.ldb = H; c.C = g.p + o_w2; c.ldc = H;
@@ -273,7 +285,7 @@ struct smc_is {
         int cur = 0, launches = 0;
         for (;;) {
             if (feats) {
-                forward(X[cur].p, rows, h1.p, h2.p, theta.p);
+                forward(X[cur].p, rows, h1.p, h2.p, thg.p, theta.p);
                 P.theta = theta.p;
                 P.theta_stride = Lpad;
                 P.feat = X[cur ^ 1].p;
@@ -341,9 +353,9 @@ struct smc_is {
         for (size_t off = 0; off < rec_pairs; off += kTrainChunk) {
             const int rows_c = (int)std::min<size_t>(kTrainChunk, rec_pairs - off);
             const uint16_t *Xc = rX.p + off * fpad;
-            forward(Xc, rows_c, th1.p, th2.p, tth.p);
+            forward(Xc, rows_c, th1.p, th2.p, tthg.p, tth.p);
             hip_check(smc::is_ce_grad(tth.p, rA.p + off * Lpad, rB.p + off * Lpad, rw.p + off, tiltable.p,
-                                      cfg.tilt_max, rows_c, Lpad, tth.p, Gb.p, st), "output gradient");
+                                      cfg.tilt_max, rows_c, Lpad, tth.p, st), "output gradient");
             backward(Xc, rows_c);
         }
     }
@@ -383,8 +395,8 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         if (c.runs_per_round == 0) c.runs_per_round = 100000;
         if (!(c.elite_frac > 0.0 && c.elite_frac < 1.0)) c.elite_frac = 0.1;
         if (c.epochs <= 0) c.epochs = 20;
-        if (!(c.learning_rate > 0.f)) c.learning_rate = 3e-3f;
-        if (!(c.tilt_max > 0.f)) c.tilt_max = 6.f;
+        if (!(c.learning_rate > 0.f)) c.learning_rate = 1e-3f;
+        if (!(c.tilt_max > 0.f)) c.tilt_max = 4.f;
         if (c.hidden % 64 || c.hidden > 1024) throw std::runtime_error("hidden width must be a multiple of 64, at most 1024");
         if (c.runs_per_round > (uint64_t)kMaxBatchRows * 8) throw std::runtime_error("limit: runs_per_round too large");
         s->cfg = c;
@@ -407,14 +419,55 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         s->fpad = (int)rup((size_t)s->F, 64);
         s->H = c.hidden;
         s->Lpad = (int)rup((size_t)s->n_locs, 64);
-        const size_t fp = s->fpad, H = s->H, L = s->Lpad;
+
+        // tilt groups: structurally identical components (same locations,
+        // delay classes, rates and edge shapes — instances of one template)
+        // share the tilt of corresponding locations; per_location gives
+        // every location its own
+        s->h_group.assign(s->n_locs, 0);
+        if (c.per_location) {
+            for (int l = 0; l < s->n_locs; ++l) s->h_group[l] = l;
+            s->n_groups = s->n_locs;
+        } else {
+            std::map<std::vector<int64_t>, int> base_of;
+            int next = 0;
+            for (int cpt = 0; cpt < img.n_comp; ++cpt) {
+                const int l0 = img.comp_first_loc[cpt], l1 = img.comp_first_loc[cpt + 1];
+                std::vector<int64_t> sig{l1 - l0};
+                for (int l = l0; l < l1; ++l) {
+                    const DevLoc &Lc = img.locs[l];
+                    int64_t rate_key;
+                    if (Lc.rate_const) memcpy(&rate_key, &Lc.rate_value, 8);
+                    else rate_key = (int64_t)Lc.rate ^ ((int64_t)1 << 62);
+                    sig.insert(sig.end(), {(int64_t)Lc.klass, (int64_t)Lc.type, (int64_t)Lc.n_inv, rate_key,
+                                           (int64_t)Lc.edge_n});
+                    for (int e = (int)Lc.edge_first; e < (int)Lc.edge_first + Lc.edge_n; ++e) {
+                        sig.push_back((int64_t)img.edges[e].dst - l0);
+                        sig.push_back((int64_t)img.edges[e].channel);
+                    }
+                }
+                auto it = base_of.find(sig);
+                int base;
+                if (it == base_of.end()) {
+                    base = next;
+                    next += l1 - l0;
+                    base_of.emplace(std::move(sig), base);
+                } else {
+                    base = it->second;
+                }
+                for (int l = l0; l < l1; ++l) s->h_group[l] = base + (l - l0);
+            }
+            s->n_groups = next;
+        }
+        s->Gpad = (int)rup((size_t)std::max(s->n_groups, 1), 64);
+        const size_t fp = s->fpad, H = s->H, L = s->Lpad, Gp = s->Gpad;
         s->o_w1 = 0;
         s->o_b1 = fp * H;
         s->o_w2 = s->o_b1 + H;
         s->o_b2 = s->o_w2 + H * H;
         s->o_w3 = s->o_b2 + H;
-        s->o_b3 = s->o_w3 + H * L;
-        s->n_params = s->o_b3 + L;
+        s->o_b3 = s->o_w3 + H * Gp;
+        s->n_params = s->o_b3 + Gp;
         s->qinit = (q->kind == SMC_QUERY_VAR && img.var_kind[q->var] == SMC_VAR_INT) ? img.var_init[q->var] : 0.0;
 
         // tiltable locations: every location that is not zero-delay
@@ -435,6 +488,26 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
 
         hipStream_t st = m->stream;
         s->tiltable.ensure(L, "hipMalloc tiltable");
+        std::vector<uint8_t> gt(Gp, 0);
+        std::vector<int32_t> mfirst(s->n_groups + 1, 0), mem(s->n_locs);
+        for (int l = 0; l < s->n_locs; ++l) {
+            gt[s->h_group[l]] |= s->h_tiltable[l];
+            ++mfirst[s->h_group[l] + 1];
+        }
+        for (int k = 0; k < s->n_groups; ++k) mfirst[k + 1] += mfirst[k];
+        {
+            std::vector<int32_t> fill(mfirst.begin(), mfirst.end() - 1);
+            for (int l = 0; l < s->n_locs; ++l) mem[fill[s->h_group[l]]++] = l;
+        }
+        s->gtiltable.ensure(Gp, "hipMalloc tiltable");
+        s->group.ensure(std::max(s->n_locs, 1), "hipMalloc groups");
+        s->member_first.ensure(mfirst.size(), "hipMalloc groups");
+        s->members.ensure(std::max(s->n_locs, 1), "hipMalloc groups");
+        hip_check(hipMemcpyAsync(s->gtiltable.p, gt.data(), Gp, hipMemcpyHostToDevice, st), "upload");
+        hip_check(hipMemcpyAsync(s->group.p, s->h_group.data(), s->n_locs * 4, hipMemcpyHostToDevice, st), "upload");
+        hip_check(hipMemcpyAsync(s->member_first.p, mfirst.data(), mfirst.size() * 4, hipMemcpyHostToDevice, st),
+                  "upload");
+        hip_check(hipMemcpyAsync(s->members.p, mem.data(), mem.size() * 4, hipMemcpyHostToDevice, st), "upload");
         s->var_scale.ensure(scale.size(), "hipMalloc scales");
         s->theta_fixed.ensure(L, "hipMalloc tilts");
         s->feat0.ensure(fp, "hipMalloc features");
@@ -467,14 +540,24 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
 
 void smc_is_destroy(smc_is *s) { delete s; }
 
-int smc_is_shape(const smc_is *s, int32_t *features_padded, int32_t *hidden, int32_t *locations_padded,
-                 uint64_t *n_params) {
+int smc_is_shape(const smc_is *s, int32_t *features_padded, int32_t *hidden, int32_t *outputs_padded,
+                 int32_t *locations_padded, uint64_t *n_params) {
     if (!s) return fail(SMC_E_INVALID, "null argument");
     if (features_padded) *features_padded = s->fpad;
     if (hidden) *hidden = s->H;
+    if (outputs_padded) *outputs_padded = s->Gpad;
     if (locations_padded) *locations_padded = s->Lpad;
     if (n_params) *n_params = s->n_params;
     return SMC_OK;
+}
+
+int smc_is_groups(const smc_is *s, int32_t *group_of_location) {
+    if (!s) {
+        fail(SMC_E_INVALID, "null argument");
+        return -1;
+    }
+    if (group_of_location) memcpy(group_of_location, s->h_group.data(), s->h_group.size() * 4);
+    return s->n_groups;
 }
 
 int smc_is_get_params(const smc_is *s, float *out) {
@@ -509,7 +592,7 @@ int smc_is_mlp_eval(smc_is *s, const float *x, int rows, const float *gout, floa
         hip_check(hipMemsetAsync(s->xf.p, 0, nx * 4, st), "clear inputs");
         hip_check(hipMemcpyAsync(s->xf.p, x, (size_t)rows * s->fpad * 4, hipMemcpyHostToDevice, st), "upload inputs");
         hip_check(smc::to_bf16(s->xf.p, s->xin.p, nx, st), "inputs to bf16");
-        s->forward(s->xin.p, rp, s->th1.p, s->th2.p, s->tth.p);
+        s->forward(s->xin.p, rp, s->th1.p, s->th2.p, s->tthg.p, s->tth.p);
         std::vector<float> th((size_t)rows * s->Lpad);
         hip_check(hipMemcpyAsync(th.data(), s->tth.p, th.size() * 4, hipMemcpyDeviceToHost, st), "copy tilts");
         hip_check(hipStreamSynchronize(st), "MLP forward");
@@ -523,7 +606,6 @@ int smc_is_mlp_eval(smc_is *s, const float *x, int rows, const float *gout, floa
                     if (s->h_tiltable[l] && std::fabs(th[i]) < s->cfg.tilt_max) gm[i] = gout[i];
                 }
             hip_check(hipMemcpyAsync(s->tth.p, gm.data(), nt * 4, hipMemcpyHostToDevice, st), "upload output gradient");
-            hip_check(smc::to_bf16(s->tth.p, s->Gb.p, nt, st), "output gradient to bf16");
             hip_check(hipMemsetAsync(s->g.p, 0, s->n_params * 4, st), "clear gradient");
             s->backward(s->xin.p, rp);
             if (grad_out)
@@ -700,7 +782,13 @@ int smc_is_train(smc_is *s, smc_is_round *rounds) {
                 for (int i : elite) lmax = std::max(lmax, l2[i]);
                 std::vector<double> wv(elite.size());
                 double wsum = 0.0;
-                for (size_t k = 0; k < elite.size(); ++k) { wv[k] = std::exp2(l2[elite[k]] - lmax); wsum += wv[k]; }
+                double wsq = 0.0;
+                for (size_t k = 0; k < elite.size(); ++k) {
+                    wv[k] = std::exp2(l2[elite[k]] - lmax);
+                    wsum += wv[k];
+                    wsq += wv[k] * wv[k];
+                }
+                info.ess = wsum * wsum / wsq;
                 const int ne = (int)elite.size();
                 const int rows = (int)rup((size_t)ne, 64);
                 std::vector<uint64_t> eruns(rows, 0);
@@ -714,7 +802,7 @@ int smc_is_train(smc_is *s, smc_is_round *rounds) {
                 rb.tilt = Tilt::MLP;
                 rb.seed = c.seed;
                 rb.n_valid = ne;
-                s->record(rb);
+                info.pairs = s->record(rb);
                 // 5. epochs over the recorded pairs
                 for (int e = 0; e < c.epochs && s->rec_pairs > 0; ++e) {
                     s->ce_gradient();

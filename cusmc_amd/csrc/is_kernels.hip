@@ -344,18 +344,42 @@ __global__ __launch_bounds__(64) void is_advance_kernel(IsParams P) {
                 if (!(d < INF)) { t = INF; end = true; break; }
                 for (int i = sl; i < m.n_clocks; i += S) vars[m.clocks[i]] += d;
                 wave_sync();
+                // the winner's transition, once, on the lane holding it
+                const int wl = seg_base + w / C;
+                int e_fired = -1, new_chan = 0;
+                if (lane == wl) {
+                    const int wj = w % C;
+                    const int lj = slot_get<C>(loc, wj);
+                    const uint64_t mkw = enabled_mask(lj, 0, vars, m);
+                    if (mkw) {
+                        const int first = (int)m.locs[lj].edge_first;
+                        const int i = pick_edge(mkw, first, P.seed, run, step, SMC_P_WINNER, (uint32_t)w, vars, m);
+                        e_fired = first + i;
+                        slot_set<C>(loc, wj, fire_edge(e_fired, vars, m, q, goal_local, new_chan));
+                    }
+                }
+                wave_sync();
+                e_fired = __shfl(e_fired, wl);
+                if (e_fired >= 0) {
+                    ++trans;
+                    digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)w << 20) | (uint64_t)e_fired);
+                    chan = __shfl(new_chan, wl);
+                    sender = chan > 0 ? w : -1;
+                    fired_any = true;
+                }
+                goal = (__ballot(goal_local) & seg_mask) != 0;
+                break;
             }
-            const int recv = phase == 0 ? chan : 0;
+            // phase 0: pending broadcast receivers
             uint64_t mk[C];
             bool any = false;
 #pragma unroll
             for (int j = 0; j < C; ++j) {
                 const int c = sl * C + j;
-                const bool sel = phase == 0 ? (c < n && c != sender) : (c == w);
-                mk[j] = sel ? enabled_mask(loc[j], recv, vars, m) : 0ull;
+                mk[j] = (c < n && c != sender) ? enabled_mask(loc[j], chan, vars, m) : 0ull;
                 any |= mk[j] != 0;
             }
-            int new_chan = 0;
+            int dummy_chan = 0;
             for (uint64_t lanes_with = __ballot(any) & seg_mask; lanes_with; lanes_with &= lanes_with - 1) {
                 const int L = lowest_lane(lanes_with);
 #pragma clang loop unroll(disable)
@@ -366,23 +390,19 @@ __global__ __launch_bounds__(64) void is_advance_kernel(IsParams P) {
                     if (lane == L) {
                         const int lj = slot_get<C>(loc, j);
                         const int first = (int)m.locs[lj].edge_first;
-                        const int i = pick_edge(slot_get<C>(mk, j), first, P.seed, run, step,
-                                                phase == 0 ? SMC_P_RECEIVER : SMC_P_WINNER, (uint32_t)c, vars, m);
+                        const int i = pick_edge(slot_get<C>(mk, j), first, P.seed, run, step, SMC_P_RECEIVER,
+                                                (uint32_t)c, vars, m);
                         e_fired = first + i;
-                        slot_set<C>(loc, j, fire_edge(e_fired, vars, m, q, goal_local, new_chan));
+                        slot_set<C>(loc, j, fire_edge(e_fired, vars, m, q, goal_local, dummy_chan));
                     }
                     wave_sync();
                     ++trans;
                     const int e = __shfl(e_fired, L);
                     digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)c << 20) | (uint64_t)e);
-                    if (phase == 1) {
-                        chan = __shfl(new_chan, L);
-                        sender = chan > 0 ? w : -1;
-                        fired_any = true;
-                    }
                 }
             }
-            if (phase == 0) { chan = 0; sender = -1; }
+            chan = 0;
+            sender = -1;
             goal = (__ballot(goal_local) & seg_mask) != 0;
         }
         mid = false;
@@ -527,8 +547,7 @@ __global__ __launch_bounds__(1024) void record_slots_kernel(const uint8_t *statu
 //   G_L = -w ln2 (A_L - B_L 2^theta_L)
 // and 0 where theta is clamped or the location is not tiltable.
 __global__ void ce_grad_kernel(const float *theta, const float *A, const float *B, const float *w,
-                               const uint8_t *tiltable, float tilt_max, int lpad, size_t n, float *G,
-                               uint16_t *Gb) {
+                               const uint8_t *tiltable, float tilt_max, int lpad, size_t n, float *G) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int l = (int)(i % (size_t)lpad);
@@ -540,7 +559,31 @@ __global__ void ce_grad_kernel(const float *theta, const float *A, const float *
         g = (float)(-(double)w[row] * LN2 * ((double)A[i] - (double)B[i] * smc_exp2((double)th)));
     }
     G[i] = g;
-    Gb[i] = f32_to_bf16(g);
+}
+
+// theta per location from the outputs of the location's tilt group.
+__global__ void expand_groups_kernel(const float *tg, int gpad, const int32_t *group, int n_locs, int lpad,
+                                     size_t n, float *tl) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int l = (int)(i % (size_t)lpad);
+    const size_t row = i / (size_t)lpad;
+    tl[i] = l < n_locs ? tg[row * gpad + group[l]] : 0.f;
+}
+
+// Gradient of a group's output: the sum over its member locations, in the
+// fixed member order (CSR member_first / members).
+__global__ void group_sum_kernel(const float *gl, int lpad, const int32_t *member_first, const int32_t *members,
+                                 int n_groups, int gpad, size_t n, float *gg, uint16_t *ggb) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int gidx = (int)(i % (size_t)gpad);
+    const size_t row = i / (size_t)gpad;
+    float s = 0.f;
+    if (gidx < n_groups)
+        for (int k = member_first[gidx]; k < member_first[gidx + 1]; ++k) s += gl[row * lpad + members[k]];
+    gg[i] = s;
+    ggb[i] = f32_to_bf16(s);
 }
 
 // Column sums over a slice of rows (blockIdx.y), 64 columns per workgroup.
@@ -743,6 +786,24 @@ hipError_t colsum(const float *X, int rows, int ld, int N, float *dst, int split
     return hipGetLastError();
 }
 
+hipError_t expand_groups(const float *tg, int gpad, const int32_t *group, int n_locs, int lpad, int rows, float *tl,
+                         hipStream_t s) {
+    const size_t n = (size_t)rows * lpad;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(expand_groups_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, tg, gpad, group,
+                       n_locs, lpad, n, tl);
+    return hipGetLastError();
+}
+
+hipError_t group_sum(const float *gl, int lpad, const int32_t *member_first, const int32_t *members, int n_groups,
+                     int gpad, int rows, float *gg, uint16_t *ggb, hipStream_t s) {
+    const size_t n = (size_t)rows * gpad;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(group_sum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, gl, lpad, member_first,
+                       members, n_groups, gpad, n, gg, ggb);
+    return hipGetLastError();
+}
+
 hipError_t is_init_rows(const IsParams &p, const uint64_t *runs, uint64_t run_begin, int n_valid,
                         const uint16_t *feat0, hipStream_t s) {
     if (p.n_rows <= 0) return hipSuccess;
@@ -757,11 +818,11 @@ hipError_t is_record_slots(const uint8_t *status, int n_rows, int32_t *pos, unsi
 }
 
 hipError_t is_ce_grad(const float *theta, const float *A, const float *B, const float *w, const uint8_t *tiltable,
-                      float tilt_max, int rows, int lpad, float *G, uint16_t *Gb, hipStream_t s) {
+                      float tilt_max, int rows, int lpad, float *G, hipStream_t s) {
     const size_t n = (size_t)rows * lpad;
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(ce_grad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, theta, A, B, w, tiltable,
-                       tilt_max, lpad, n, G, Gb);
+                       tilt_max, lpad, n, G);
     return hipGetLastError();
 }
 

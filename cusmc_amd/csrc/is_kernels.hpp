@@ -78,7 +78,13 @@ hipError_t is_record_slots(const uint8_t *status, int n_rows, int32_t *pos, unsi
                            hipStream_t s);
 // Cross-entropy gradient wrt the MLP outputs (see is_kernels.hip).
 hipError_t is_ce_grad(const float *theta, const float *A, const float *B, const float *w, const uint8_t *tiltable,
-                      float tilt_max, int rows, int lpad, float *G, uint16_t *Gb, hipStream_t s);
+                      float tilt_max, int rows, int lpad, float *G, hipStream_t s);
+// tl[r][l] = tg[r][group[l]] (0 for padding locations).
+hipError_t expand_groups(const float *tg, int gpad, const int32_t *group, int n_locs, int lpad, int rows, float *tl,
+                         hipStream_t s);
+// gg[r][g] = sum of gl[r][l] over the members l of group g (fp32 and bf16).
+hipError_t group_sum(const float *gl, int lpad, const int32_t *member_first, const int32_t *members, int n_groups,
+                     int gpad, int rows, float *gg, uint16_t *ggb, hipStream_t s);
 hipError_t gemm(const GemmArgs &g, bool ta, bool tb, hipStream_t s);
 // dst (+)= op(A) op(B) with K split over `splits` workgroup layers (partial
 // sums in `work`, splits * M * N floats, folded in a fixed order).

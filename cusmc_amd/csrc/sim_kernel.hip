@@ -248,21 +248,45 @@ __global__ __launch_bounds__(64) void sim_kernel(LaunchParams lp) {
                     // clocks advance (find_minimum_delay :632-651)
                     for (int i = sl; i < m.n_clocks; i += S) vars[m.clocks[i]] += d;
                     wave_sync();
+                    // the winner's transition (check_enabled_edges :268-305,
+                    // take_transition :27-181), once, on the lane holding it
+                    const int wl = seg_base + w / C;
+                    int e_fired = -1, new_chan = 0;
+                    if (lane == wl) {
+                        const int wj = w % C;
+                        const int lj = slot_get<C>(loc, wj);
+                        const uint64_t mkw = enabled_mask(lj, 0, vars, m);
+                        if (mkw) {
+                            const int first = (int)m.locs[lj].edge_first;
+                            const int i = pick_edge(mkw, first, p.seed, run, step, SMC_P_WINNER, (uint32_t)w, vars, m);
+                            e_fired = first + i;
+                            slot_set<C>(loc, wj, fire_edge(e_fired, vars, m, q, goal_local, new_chan));
+                        }
+                    }
+                    wave_sync();
+                    e_fired = __shfl(e_fired, wl);
+                    if (e_fired >= 0) {
+                        ++trans;
+                        if (TRACE)
+                            digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)w << 20) | (uint64_t)e_fired);
+                        chan = __shfl(new_chan, wl);
+                        sender = chan > 0 ? w : -1;
+                    }
+                    goal = (__ballot(goal_local) & seg_mask) != 0;
+                    break;
                 }
 
-                // enabled edges of the components that may fire in this phase
-                const int recv = phase == 0 ? chan : 0;
+                // phase 0: enabled receive edges of every component but the sender
                 uint64_t mk[C];
                 bool any = false;
 #pragma unroll
                 for (int j = 0; j < C; ++j) {
                     const int c = sl * C + j;
-                    const bool sel = phase == 0 ? (c < n && c != sender) : (c == w);
-                    mk[j] = sel ? enabled_mask(loc[j], recv, vars, m) : 0ull;
+                    mk[j] = (c < n && c != sender) ? enabled_mask(loc[j], chan, vars, m) : 0ull;
                     any |= mk[j] != 0;
                 }
                 // fire them one at a time in component order
-                int new_chan = 0;
+                int dummy_chan = 0;
                 for (uint64_t lanes_with = __ballot(any) & seg_mask; lanes_with; lanes_with &= lanes_with - 1) {
                     const int L = lowest_lane(lanes_with);
 #pragma clang loop unroll(disable)
@@ -273,11 +297,10 @@ __global__ __launch_bounds__(64) void sim_kernel(LaunchParams lp) {
                         if (lane == L) {
                             const int lj = slot_get<C>(loc, j);
                             const int first = (int)m.locs[lj].edge_first;
-                            const int i = pick_edge(slot_get<C>(mk, j), first, p.seed, run, step,
-                                                    phase == 0 ? SMC_P_RECEIVER : SMC_P_WINNER,
+                            const int i = pick_edge(slot_get<C>(mk, j), first, p.seed, run, step, SMC_P_RECEIVER,
                                                     (uint32_t)c, vars, m);
                             e_fired = first + i;
-                            slot_set<C>(loc, j, fire_edge(e_fired, vars, m, q, goal_local, new_chan));
+                            slot_set<C>(loc, j, fire_edge(e_fired, vars, m, q, goal_local, dummy_chan));
                         }
                         wave_sync();
                         ++trans;
@@ -285,13 +308,10 @@ __global__ __launch_bounds__(64) void sim_kernel(LaunchParams lp) {
                             const int e = __shfl(e_fired, L);
                             digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)c << 20) | (uint64_t)e);
                         }
-                        if (phase == 1) {
-                            chan = __shfl(new_chan, L);
-                            sender = chan > 0 ? w : -1;
-                        }
                     }
                 }
-                if (phase == 0) { chan = 0; sender = -1; }
+                chan = 0;
+                sender = -1;
                 goal = (__ballot(goal_local) & seg_mask) != 0;
             }
             if (!end) { t = t + d; ++step; }

@@ -229,18 +229,21 @@ class ImportanceSampler:
 
     def __init__(self, sim: GpuSimulator, query: A.Query, *, hidden: int = 128, rounds: int = 5,
                  runs_per_round: int = 100000, elite_frac: float = 0.1, epochs: int = 20,
-                 learning_rate: float = 3e-3, tilt_max: float = 6.0, seed: int = 1):
+                 learning_rate: float = 1e-3, tilt_max: float = 4.0, per_location: bool = False, seed: int = 1):
         self._h = None
         self.sim = sim
         self.query = query
         self.config = A.IsConfig(int(hidden), int(rounds), int(runs_per_round), float(elite_frac), int(epochs),
-                                 float(learning_rate), float(tilt_max), 0, int(seed))
+                                 float(learning_rate), float(tilt_max), int(bool(per_location)), int(seed))
         h = C.c_void_p()
         A.check(A.lib().smc_is_create(sim._h, C.byref(query), C.byref(self.config), C.byref(h)))
         self._h = h
-        fp, hd, lp, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_uint64()
-        A.check(A.lib().smc_is_shape(self._h, C.byref(fp), C.byref(hd), C.byref(lp), C.byref(n)))
-        self.features_padded, self.hidden, self.locations_padded, self.n_params = fp.value, hd.value, lp.value, n.value
+        fp, hd, op, lp, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_uint64()
+        A.check(A.lib().smc_is_shape(self._h, C.byref(fp), C.byref(hd), C.byref(op), C.byref(lp), C.byref(n)))
+        self.features_padded, self.hidden, self.outputs_padded = fp.value, hd.value, op.value
+        self.locations_padded, self.n_params = lp.value, n.value
+        self.group = np.zeros(max(sim.network.net.n_locations, 1), dtype=np.int32)
+        self.n_groups = A.lib().smc_is_groups(self._h, self.group.ctypes.data_as(C.POINTER(C.c_int32)))
 
     def close(self) -> None:
         if getattr(self, "_h", None) and A._lib is not None:
@@ -254,7 +257,8 @@ class ImportanceSampler:
         """Cross-entropy rounds; returns one dict per round."""
         rounds = (A.IsRound * self.config.rounds)()
         A.check(A.lib().smc_is_train(self._h, rounds))
-        return [dict(level=r.level, elites=r.elites, hits=r.hits, estimate=r.estimate, ms=r.ms) for r in rounds]
+        return [dict(level=r.level, elites=r.elites, hits=r.hits, estimate=r.estimate, ms=r.ms, ess=r.ess,
+                     pairs=r.pairs) for r in rounds]
 
     def estimate(self, *, seed: int, run_begin: int = 0, batch_runs: int = 65536, max_runs: int = 0,
                  rel_target: float = 0.1) -> IsResult:

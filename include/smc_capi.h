@@ -185,9 +185,11 @@ typedef struct smc_is_config {
     uint64_t runs_per_round; /* trajectories per round (default 100000)         */
     double elite_frac;       /* fraction kept per round (default 0.1)           */
     int32_t epochs;          /* gradient steps per round (default 20)           */
-    float learning_rate;     /* Adam step (default 3e-3)                        */
-    float tilt_max;          /* bound on |theta| (log2 units, default 6)        */
-    int32_t reserved;
+    float learning_rate;     /* Adam step (default 1e-3)                        */
+    float tilt_max;          /* bound on |theta| (log2 units, default 4)        */
+    int32_t per_location;    /* 1: a tilt per location; 0 (default): one per
+                                template location, shared by the structurally
+                                identical components of the network          */
     uint64_t seed;           /* weight initialisation and sampling streams      */
 } smc_is_config;
 
@@ -197,6 +199,8 @@ typedef struct smc_is_round {
     uint64_t hits;           /* runs reaching the query threshold               */
     double estimate;         /* IS estimate from this round's runs              */
     double ms;               /* sampling + training time                        */
+    double ess;              /* effective sample size of the elite weights      */
+    uint64_t pairs;          /* recorded (run, macro-step) pairs trained on     */
 } smc_is_round;
 
 typedef struct smc_is_result {
@@ -232,15 +236,19 @@ int smc_is_sample(smc_is *s, const float *theta, int use_mlp, uint64_t seed, uin
                   int lockstep, int early_stop, smc_is_outputs *out, smc_is_result *res);
 /* Parameter count, and flat FP32 parameters [W1 b1 W2 b2 W3 b3] (W row-major
  * in x out, rows of W1 padded to features_padded, columns of W3 to
- * locations_padded). */
-int smc_is_shape(const smc_is *s, int32_t *features_padded, int32_t *hidden, int32_t *locations_padded,
-                 uint64_t *n_params);
+ * outputs_padded: one output per tilt group). */
+int smc_is_shape(const smc_is *s, int32_t *features_padded, int32_t *hidden, int32_t *outputs_padded,
+                 int32_t *locations_padded, uint64_t *n_params);
+/* Tilt group of every location (the MLP output it reads); returns the
+ * number of groups, or -1. */
+int smc_is_groups(const smc_is *s, int32_t *group_of_location);
 int smc_is_get_params(const smc_is *s, float *out);
 int smc_is_set_params(smc_is *s, const float *in);
 /* MLP on `rows` feature rows x (rows x features_padded, FP32, rounded to
- * BF16): theta (rows x locations_padded); with grad_out != NULL also the
- * gradient of sum(g * theta_pre_clamp) over the parameters (g is rows x
- * locations_padded; entries at clamped or untiltable outputs are ignored). */
+ * BF16): theta per location (rows x locations_padded, each location reading
+ * its group's output); with g != NULL also the gradient of sum(g * theta)
+ * over the parameters (g is rows x locations_padded; entries at clamped or
+ * untiltable outputs are ignored). */
 int smc_is_mlp_eval(smc_is *s, const float *x, int rows, const float *g, float *theta, float *grad_out);
 /* Replay runs[0..n_runs) (seed) under the fixed tilt theta[location], or the
  * MLP when theta == NULL, recording every macro-step (a macro-step ends with

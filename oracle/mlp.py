@@ -10,7 +10,10 @@ accumulates in fp32; `forward`/`backward` below do the same roundings in
 float64 arithmetic, so the two agree to fp32 accumulation-order error.
 
 Parameter layout (flat fp32): W1 [fpad x H], b1 [H], W2 [H x H], b2 [H],
-W3 [H x lpad], b3 [lpad] — each W row-major (inputs x outputs).
+W3 [H x gpad], b3 [gpad] — each W row-major (inputs x outputs).  The network
+has one output per tilt group; location l reads output group[l] (`expand`),
+and the gradient of a group output is the sum over its locations
+(`group_sum`).
 """
 from __future__ import annotations
 
@@ -115,3 +118,19 @@ def ce_output_grad(theta, A, B, w, tiltable, tilt_max):
     g[np.abs(th) >= tilt_max] = 0.0
     g[:, ~np.asarray(tiltable, dtype=bool)] = 0.0
     return g
+
+
+def expand(theta_group, group, lpad):
+    """theta per location (padding locations 0)."""
+    tg = np.asarray(theta_group)
+    out = np.zeros((tg.shape[0], lpad), dtype=tg.dtype)
+    out[:, :len(group)] = tg[:, np.asarray(group)]
+    return out
+
+
+def group_sum(g_loc, group, gpad):
+    """Gradient of each group output: sum over its member locations."""
+    g = np.asarray(g_loc, dtype=np.float64)
+    out = np.zeros((g.shape[0], gpad))
+    np.add.at(out.T, np.asarray(group), g[:, :len(group)].T)
+    return out

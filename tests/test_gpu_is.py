@@ -129,7 +129,7 @@ def test_recorded_pairs_match_oracle(env, name):
 
 def _random_params(s, seed, w3_scale=0.3):
     rng = np.random.default_rng(seed)
-    fp, H, lp = s.features_padded, s.hidden, s.locations_padded
+    fp, H, lp = s.features_padded, s.hidden, s.outputs_padded
     p = np.concatenate([rng.normal(0, math.sqrt(2.0 / fp), fp * H), rng.normal(0, 0.1, H),
                         rng.normal(0, math.sqrt(2.0 / H), H * H), rng.normal(0, 0.1, H),
                         rng.normal(0, w3_scale, H * lp), rng.normal(0, 0.3, lp)]).astype(np.float32)
@@ -145,6 +145,41 @@ def _tiltable(net, s):
     return t
 
 
+def _group_tiltable(net, s):
+    t = _tiltable(net, s)
+    gt = np.zeros(s.outputs_padded, dtype=bool)
+    for l in range(net.net.n_locations):
+        gt[s.group[l]] |= t[l]
+    return gt
+
+
+def _np_theta(s, net, p, x):
+    """NumPy MLP: group outputs, then theta per location."""
+    thg, cache = mlp.forward(p, x, s.features_padded, s.hidden, s.outputs_padded, s.config.tilt_max,
+                             _group_tiltable(net, s))
+    return mlp.expand(thg, s.group[:net.net.n_locations], s.locations_padded), cache
+
+
+def test_tilt_groups_follow_templates(env):
+    net, q, sim, s = env["sir30"]
+    names = [net.location_name(l) for l in range(net.net.n_locations)]
+    assert s.n_groups == 3
+    for l, nm in enumerate(names):
+        assert s.group[l] == {"S": 0, "I": 1, "R": 2}[nm]
+    net, q, sim, s = env["fischer6"]
+    # pid differs per process, the shape does not; P0 holds the goal location
+    assert s.n_groups == 8
+    nl = net.net.n_locations
+    assert list(s.group[:4]) == [0, 1, 2, 3]
+    assert all(list(s.group[4 * k:4 * k + 4]) == [4, 5, 6, 7] for k in range(1, nl // 4))
+    p = cs.ImportanceSampler(sim, q, per_location=True, runs_per_round=1024, rounds=1)
+    try:
+        assert p.n_groups == net.net.n_locations
+        assert np.array_equal(p.group[:net.net.n_locations], np.arange(net.net.n_locations))
+    finally:
+        p.close()
+
+
 def test_mlp_forward_backward_matches_numpy(env):
     net, q, sim, s = env["sir30"]
     p = _random_params(s, 11)
@@ -155,14 +190,15 @@ def test_mlp_forward_backward_matches_numpy(env):
     g = rng.normal(0.0, 1.0, (rows, s.locations_padded)).astype(np.float32)
     tl = _tiltable(net, s)
     theta, grad = s.mlp_eval(x, g)
-    th_np, cache = mlp.forward(p, x, s.features_padded, s.hidden, s.locations_padded, s.config.tilt_max, tl)
+    th_np, cache = _np_theta(s, net, p, x)
     err = np.abs(theta - th_np)
     print(f"[mlp] theta max err {err.max():.3e} norm {np.linalg.norm(th_np):.3e}", flush=True)
     assert np.linalg.norm(theta - th_np) <= 2e-3 * np.linalg.norm(th_np)
     assert np.all(theta[:, ~tl] == 0.0)
     assert np.all(np.abs(theta) <= s.config.tilt_max)
     gm = np.where(tl[None, :] & (np.abs(theta) < s.config.tilt_max), g, 0.0)
-    g_np = mlp.backward(p, cache, gm, s.features_padded, s.hidden, s.locations_padded)
+    gg = mlp.group_sum(gm, s.group[:net.net.n_locations], s.outputs_padded)
+    g_np = mlp.backward(p, cache, gg, s.features_padded, s.hidden, s.outputs_padded)
     rel = np.linalg.norm(grad - g_np) / np.linalg.norm(g_np)
     print(f"[mlp] grad rel err {rel:.3e}", flush=True)
     assert rel < 2e-2
@@ -191,9 +227,10 @@ def test_ce_gradient_matches_oracle(env):
     xo = mlp.bf16_bits_to_f32(np.array(ox))
     assert np.array_equal(X, np.array(ox))
     tl = _tiltable(net, s)
-    th_np, cache = mlp.forward(p, xo, s.features_padded, s.hidden, s.locations_padded, s.config.tilt_max, tl)
+    th_np, cache = _np_theta(s, net, p, xo)
     G_np = mlp.ce_output_grad(th_np, np.array(oa), np.array(ob), np.array(ow), tl, s.config.tilt_max)
-    g_np = mlp.backward(p, cache, G_np, s.features_padded, s.hidden, s.locations_padded)
+    gg = mlp.group_sum(G_np, s.group[:net.net.n_locations], s.outputs_padded)
+    g_np = mlp.backward(p, cache, gg, s.features_padded, s.hidden, s.outputs_padded)
     rel = np.linalg.norm(grad - g_np) / np.linalg.norm(g_np)
     cos = float(grad @ g_np / (np.linalg.norm(grad) * np.linalg.norm(g_np)))
     print(f"[ce grad] pairs={len(ox)} rel err {rel:.3e} cos {cos:.6f}", flush=True)
