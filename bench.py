@@ -15,6 +15,7 @@ Other workloads (--workload):
            Pr[<=50](<> I>=90): wall time to a +-10% 95% interval with the
            learned importance sampler (training included) against the plain
            simulator
+  copy     a 1 GiB device copy: the calibration of HBM byte counters
 
 roofline: the run kernel is bound by VALU issue (tables sit in L2, per-run
 output is a few bytes), so `achieved` counts algorithmic lane-operations:
@@ -65,13 +66,16 @@ def bench_plain(args) -> dict:
     net, q, runs, name = _models(args.workload)
     runs = args.runs or runs
     sim = cs.GpuSimulator(net, device=0)
-    tuned = sim.autotune(q, seed=SEED + 999, sample_runs=min(runs, 65536))
+    tuned = sim.autotune(q, seed=SEED + 999, sample_runs=max(1024, min(runs // 4, 16384)))
+    _progress(f"autotune: {tuned.lanes_per_run} lanes per run")
     for w in range(args.warmup):
         sim.run(q, seed=SEED + 1000 + w, n_runs=runs)
+        _progress(f"warmup {w + 1}/{args.warmup}")
     results = []
     t0 = time.perf_counter()
     for k in range(args.steps):
         results.append(sim.run(q, seed=SEED + k, n_runs=runs))
+        _progress(f"step {k + 1}/{args.steps}: {results[-1].kernel_ms:.1f} ms kernel")
     wall = time.perf_counter() - t0
     sim.close()
     n = sum(r.n_runs for r in results)
@@ -128,8 +132,10 @@ def bench_batch(args) -> dict:
     t_build = time.perf_counter()
     b = cs.Batch(items, tune=True)
     t_build = time.perf_counter() - t_build
+    _progress(f"batch of {len(items)} instances captured in {t_build:.1f} s")
     for _ in range(args.warmup):
         b.run(1)
+        _progress("warmup round")
     t0 = time.perf_counter()
     gms = 0.0
     tot_runs = tot_trans = tot_steps = 0
@@ -139,6 +145,7 @@ def bench_batch(args) -> dict:
         tot_runs += st.runs
         tot_trans += st.transitions
         tot_steps += st.steps
+        _progress(f"round: {st.graph_ms:.1f} ms")
     wall = time.perf_counter() - t0
     groups = st.groups
     b.close()
@@ -177,6 +184,7 @@ def bench_is(args) -> dict:
         n += r.n_runs
         p = hits / n
         rel = 1.959963984540054 * math.sqrt(p * (1 - p) / n) / p if hits else math.inf
+        print(f"plain: {n} runs, {hits} hits, rel half-width {rel:.3f}", file=sys.stderr, flush=True)
         if hits >= 10 and rel <= 0.1:
             break
     plain_s = time.perf_counter() - t0
@@ -214,18 +222,70 @@ def bench_is(args) -> dict:
     }
 
 
+def _progress(msg: str) -> None:
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def _host_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(net, q, args) -> dict:
-    """The oracle (cuSMC's per-component race, literal mode) on all host
-    cores, on a bounded sample of the same workload."""
+    """The oracle (cuSMC's per-component race, literal mode) on the host
+    cores, on a bounded sample of the same workload: all cores the process
+    may use, and one core."""
     import oracle
-    threads = os.cpu_count() or 1
-    n = 256
-    res, _ = oracle.run(net, q, seed=SEED, n_runs=n, mode=oracle.LITERAL, threads=threads)
-    rate = n / max(res.seconds, 1e-9)
-    n = max(threads, int(rate * args.cpu_seconds))
-    res, _ = oracle.run(net, q, seed=SEED, n_runs=n, mode=oracle.LITERAL, threads=threads)
+
+    def timed(threads, seconds):
+        n = max(threads, 64)
+        res, _ = oracle.run(net, q, seed=SEED, n_runs=n, mode=oracle.LITERAL, threads=threads)
+        rate = n / max(res.seconds, 1e-9)
+        n = max(threads, int(rate * seconds))
+        res, _ = oracle.run(net, q, seed=SEED, n_runs=n, mode=oracle.LITERAL, threads=threads)
+        return n, res
+    threads = _host_cores()
+    n, res = timed(threads, args.cpu_seconds)
+    n1, res1 = timed(1, max(2.0, args.cpu_seconds / 3))
+    _progress(f"cpu baseline: {n / res.seconds:.1f} runs/s on {res.threads} threads")
     return {"value": n / res.seconds, "unit": "runs/s", "cores": res.threads, "kind": "port",
-            "sample": f"{n} runs of the same workload, oracle literal mode, {res.seconds:.1f} s"}
+            "sample": f"{n} runs of the same workload, oracle literal mode, {res.seconds:.1f} s on "
+                      f"{res.threads} threads ({_cpu_model()})",
+            "single_core": {"value": n1 / res1.seconds, "unit": "runs/s", "sample": f"{n1} runs, {res1.seconds:.1f} s"}}
+
+
+def bench_copy(args) -> dict:
+    """A plain device copy of a known size: the calibration kernel for the
+    HBM byte counters (FETCH_SIZE, WRITE_SIZE) of the profiles."""
+    import torch
+    nbytes = 1 << 30
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"metric": "device copy bandwidth (read + write)", "value": 2 * nbytes * args.steps / wall / 1e9,
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"copy {nbytes} bytes"}}
 
 
 def main() -> int:
@@ -233,7 +293,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="fischer32", choices=["fischer32", "sir350", "race8", "batch", "is"])
+    ap.add_argument("--workload", default="fischer32",
+                    choices=["fischer32", "sir350", "race8", "batch", "is", "copy"])
     ap.add_argument("--runs", type=int, default=0, help="runs per step (default: the workload's)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--plain-cap-s", type=float, default=120.0, help="is: cap on the plain run's wall time")
@@ -244,7 +305,9 @@ def main() -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not run here (the run range would be split "
               f"across GPUs, one process each, with a reduction of a few scalars)", file=sys.stderr)
         return 2
-    if args.workload == "batch":
+    if args.workload == "copy":
+        line = bench_copy(args)
+    elif args.workload == "batch":
         line = bench_batch(args)
     elif args.workload == "is":
         line = bench_is(args)

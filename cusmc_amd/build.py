@@ -29,6 +29,9 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", 
 
 HOST_SOURCES = ["network.cpp", "uppaal.cpp", "compile.cpp"]
 HIP_SOURCES = ["capi.cpp", "is.cpp", "sim_kernel.hip", "is_kernels.hip"]
+# kernels compiled once per (component slots per lane, register policy)
+VARIANT_SOURCES = ["sim_variant.hip", "is_variant.hip"]
+VARIANTS = [(c, o) for c in (1, 2, 4, 8, 16) for o in (0, 2)]
 
 
 def _run(cmd: list[str]) -> None:
@@ -42,21 +45,33 @@ def _obj(src: str) -> Path:
     return BUILD / (src.replace(".", "_") + ".o")
 
 
+def _variant_obj(src: str, c: int, o: int) -> Path:
+    return BUILD / f"{src.replace('.', '_')}_c{c}_o{o}.o"
+
+
 def build_lib() -> Path:
     BUILD.mkdir(parents=True, exist_ok=True)
     LIB.mkdir(parents=True, exist_ok=True)
-    jobs = []
-    for s in HOST_SOURCES:
-        jobs.append([CXX, *COMMON, "-Wall", "-c", str(CSRC / s), "-o", str(_obj(s))])
+    jobs, objs = [], []
+    for s in VARIANT_SOURCES:  # the longest jobs first
+        for c, o in sorted(VARIANTS, key=lambda v: -v[0]):
+            obj = _variant_obj(s, c, o)
+            jobs.append([HIPCC, *COMMON, *HIP_FLAGS, "-x", "hip", f"-DSMC_C={c}", f"-DSMC_OCC={o}", "-c",
+                         str(CSRC / s), "-o", str(obj)])
+            objs.append(obj)
     for s in HIP_SOURCES:
         extra = ["-x", "hip"] if s.endswith(".hip") else []
         jobs.append([HIPCC, *COMMON, *HIP_FLAGS, *extra, "-c", str(CSRC / s), "-o", str(_obj(s))])
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        objs.append(_obj(s))
+    for s in HOST_SOURCES:
+        jobs.append([CXX, *COMMON, "-Wall", "-c", str(CSRC / s), "-o", str(_obj(s))])
+        objs.append(_obj(s))
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "0")) or os.cpu_count() or 8))
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         list(ex.map(_run, jobs))
     out = LIB / "libcusmc.so"
     tmp = LIB / "libcusmc.so.tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-          *[str(_obj(s)) for s in HOST_SOURCES + HIP_SOURCES]])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *[str(o) for o in objs]])
     os.replace(tmp, out)
     return out
 

@@ -10,6 +10,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "capi_internal.hpp"
@@ -30,15 +31,18 @@ namespace {
 // Launch shape for one group of instances.
 struct Plan {
     int S = 1, C = 1, R = 64, stride = 1;
+    int occ = 0;  // run-kernel build: 0 registers unrestricted, 2 two waves per SIMD
     size_t lds = 0;
     int waves = 1;
     uint32_t chunk_runs = 64;
     uint64_t total_chunks = 0;
 };
 
-Plan plan_launch(int P, int max_vars, int lanes, int waves_req, int waves_per_cu, uint64_t total_runs,
+Plan plan_launch(int P, int max_vars, int lanes, int occ, int waves_req, int waves_per_cu, uint64_t total_runs,
                  uint64_t max_runs_per_inst, bool trace, int cu_count) {
     Plan pl;
+    if (occ != 0 && occ != 2) throw std::runtime_error("occupancy must be 0 or 2");
+    pl.occ = occ;
     int S = lanes ? lanes : (P < 64 ? P : 64);
     if (S <= 0 || S > 64 || (S & (S - 1))) throw std::runtime_error("lanes_per_run must be a power of two <= 64");
     if (S > P) S = P;
@@ -47,12 +51,13 @@ Plan plan_launch(int P, int max_vars, int lanes, int waves_req, int waves_per_cu
     if (pl.C > 16) throw std::runtime_error("limit: more than 16 components per lane; use more lanes per run");
     pl.R = 64 / S;
     pl.stride = (max_vars > 0 ? max_vars : 1) | 1;
-    pl.lds = (size_t)pl.R * pl.stride * sizeof(double);
+    // variables of the R packed runs, then 12 doubles of race-tree levels per lane
+    pl.lds = (size_t)pl.R * pl.stride * sizeof(double) + 64 * 12 * sizeof(double);
     if (pl.lds > 60000) throw std::runtime_error("limit: variables of the packed runs exceed the LDS budget");
     int waves = waves_req;
     if (waves <= 0) {
         int per_cu = 0;
-        hip_check(smc::sim_occupancy(pl.C, trace, pl.lds, &per_cu), "occupancy query");
+        hip_check(smc::sim_occupancy(pl.C, trace, pl.occ, pl.lds, &per_cu), "occupancy query");
         if (waves_per_cu > 0 && waves_per_cu < per_cu) per_cu = waves_per_cu;
         if (per_cu <= 0) per_cu = 1;
         waves = cu_count * per_cu;
@@ -262,10 +267,12 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
         if (n_runs > (1ull << 32) - 1) throw std::runtime_error("limit: at most 2^32-1 runs per call");
         if (run_begin > (~0ull) - n_runs) throw std::runtime_error("run range overflows");
         const bool trace = outputs && outputs->digest;
-        const smc_launch *use = (launch && (launch->lanes_per_run || launch->waves || launch->waves_per_cu))
-                                    ? launch : (m->has_tuned ? &m->tuned : nullptr);
+        const smc_launch *use =
+            (launch && (launch->lanes_per_run || launch->waves || launch->waves_per_cu || launch->occupancy))
+                ? launch : (m->has_tuned ? &m->tuned : nullptr);
         hip_check(hipSetDevice(m->device), "hipSetDevice");
-        Plan pl = plan_launch(m->img.n_leaves, m->img.n_vars, use ? use->lanes_per_run : 0, use ? use->waves : 0,
+        Plan pl = plan_launch(m->img.n_leaves, m->img.n_vars, use ? use->lanes_per_run : 0, use ? use->occupancy : 0,
+                              use ? use->waves : 0,
                               use ? use->waves_per_cu : 0, n_runs, n_runs, trace, m->cu_count);
         m->runs.ensure(n_runs);
         pl.total_chunks = (n_runs + pl.chunk_runs - 1) / pl.chunk_runs;
@@ -289,7 +296,7 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
         hip_check(hipMemcpyAsync(m->d_inst, &m->h_inst, sizeof(InstDesc), hipMemcpyHostToDevice, m->stream), "upload instance");
         hip_check(smc::launch_prep(m->d_counter, nullptr, 0, m->stream), "prep kernel launch");
         hip_check(hipEventRecord(m->ev[0], m->stream), "event");
-        hip_check(smc::launch_sim(lp, pl.C, trace, waves, pl.lds, m->stream), "run kernel launch");
+        hip_check(smc::launch_sim(lp, pl.C, trace, pl.occ, waves, pl.lds, m->stream), "run kernel launch");
         hip_check(hipEventRecord(m->ev[1], m->stream), "event");
         hip_check(smc::launch_reduce(m->d_inst, 1, blocks, m->d_partials, m->d_acc, 0, m->stream), "reduce kernel launch");
         hip_check(hipEventRecord(m->ev[2], m->stream), "event");
@@ -325,18 +332,29 @@ int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sampl
         std::vector<smc_launch> cands;
         for (int S = P < 64 ? P : 64; S >= 1; S >>= 1) {
             if (P / S > 16) break;
-            cands.push_back(smc_launch{S, 0, 0, 0});
+            for (int occ : {0, 2}) cands.push_back(smc_launch{S, 0, 0, occ});
         }
-        double best_ms = 1e300;
+        // each candidate runs at least twice the runs its grid holds at once,
+        // so the sample measures the steady state rather than the tail
+        double best_cost = 1e300;
         smc_launch pick = cands.front();
         for (const smc_launch &c : cands) {
+            uint64_t n = sample_runs;
+            try {
+                Plan pl = plan_launch(P, m->img.n_vars, c.lanes_per_run, c.occupancy, 0, 0, sample_runs, sample_runs,
+                                      false, m->cu_count);
+                n = std::max<uint64_t>(n, 2ull * (uint64_t)pl.waves * (uint64_t)pl.R);
+            } catch (const std::exception &) {
+                continue;  // e.g. LDS budget exceeded at this packing
+            }
             smc_result r;
-            int rc = smc_run(m, q, seed, 0, sample_runs, &c, nullptr, &r);
-            if (rc != SMC_OK) continue;  // e.g. LDS budget exceeded at this packing
-            rc = smc_run(m, q, seed + 1, sample_runs, sample_runs, &c, nullptr, &r);
-            if (rc == SMC_OK && r.kernel_ms < best_ms) { best_ms = r.kernel_ms; pick = c; }
+            int rc = smc_run(m, q, seed, 0, std::min<uint64_t>(n, 4096), &c, nullptr, &r);  // warm
+            if (rc != SMC_OK) continue;
+            rc = smc_run(m, q, seed + 1, 0, n, &c, nullptr, &r);
+            const double cost = r.kernel_ms / (double)n;
+            if (rc == SMC_OK && cost < best_cost) { best_cost = cost; pick = c; }
         }
-        if (!(best_ms < 1e300)) throw std::runtime_error("no launch shape could run this model");
+        if (!(best_cost < 1e300)) throw std::runtime_error("no launch shape could run this model");
         m->tuned = pick;
         m->has_tuned = true;
         if (best) *best = pick;
@@ -407,8 +425,8 @@ int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_bat
         const int cu = items[0].model->cu_count;
 
         // group by launch shape: tuned (per size class) or default
-        std::map<std::pair<int, int>, std::vector<int>> by_shape;  // (P, S) -> members
-        std::map<int, int> tuned_S;                                 // P -> S
+        std::map<std::tuple<int, int, int>, std::vector<int>> by_shape;  // (P, S, occupancy) -> members
+        std::map<int, smc_launch> tuned;                            // P -> lanes, occupancy
         if (tune) {
             std::map<int, int> rep;  // P -> item with the most components... first seen
             for (int i = 0; i < n_items; ++i) {
@@ -422,15 +440,16 @@ int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_bat
                 uint64_t sample = std::min<uint64_t>(it.n_runs, 16384);
                 if (smc_autotune(it.model, &it.query, it.seed, sample, &best) != SMC_OK)
                     throw std::runtime_error(std::string("autotune failed: ") + g_last_error);
-                tuned_S[kv.first] = best.lanes_per_run;
+                tuned[kv.first] = best;
             }
         }
         for (int i = 0; i < n_items; ++i) {
             const smc_model *m = items[i].model;
             const int P = m->img.n_leaves;
-            int S = tuned_S.count(P) ? tuned_S[P] : (m->has_tuned ? m->tuned.lanes_per_run : 0);
+            const smc_launch *t = tuned.count(P) ? &tuned[P] : (m->has_tuned ? &m->tuned : nullptr);
+            int S = t ? t->lanes_per_run : 0;
             if (!S) S = P < 64 ? P : 64;
-            by_shape[{P, S}].push_back(i);
+            by_shape[{P, S, t ? t->occupancy : 0}].push_back(i);
         }
         b->bufs.resize(n_items);
         std::vector<InstDesc> all(n_items);
@@ -444,7 +463,8 @@ int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_bat
                 maxr = std::max<uint64_t>(maxr, items[i].n_runs);
                 max_vars = std::max(max_vars, items[i].model->img.n_vars);
             }
-            g.pl = plan_launch(kv.first.first, max_vars, kv.first.second, 0, 0, total, maxr, false, cu);
+            g.pl = plan_launch(std::get<0>(kv.first), max_vars, std::get<1>(kv.first), std::get<2>(kv.first), 0, 0, total,
+                               maxr, false, cu);
             std::vector<InstDesc> inst;
             uint64_t chunk = 0;
             for (int i : g.members) {
@@ -500,7 +520,8 @@ int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_bat
         for (size_t k = 0; k < b->groups.size(); ++k) {
             auto &g = b->groups[k];
             hip_check(hipStreamWaitEvent(b->gstreams[k], b->fork, 0), "fork wait");
-            hip_check(smc::launch_sim(g.lp, g.pl.C, false, g.pl.waves, g.pl.lds, b->gstreams[k]), "capture run kernel");
+            hip_check(smc::launch_sim(g.lp, g.pl.C, false, g.pl.occ, g.pl.waves, g.pl.lds, b->gstreams[k]),
+                      "capture run kernel");
             hip_check(hipEventRecord(b->joins[k], b->gstreams[k]), "join");
             hip_check(hipStreamWaitEvent(b->stream, b->joins[k], 0), "join wait");
         }

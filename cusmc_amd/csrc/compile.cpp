@@ -20,6 +20,8 @@
 #include <cstring>
 #include <map>
 #include <stdexcept>
+#include <algorithm>
+#include <cmath>
 #include <string>
 #include <unordered_map>
 #include "expr_ops.h"
@@ -188,6 +190,59 @@ ModelImage compile_model(const smc_network &n) {
         }
         img.locs.push_back(d);
     }
+    // Receive edges whose updates commute within one broadcast.  Forms of
+    // an update: ADD (integer variable += integer constant), SET (:= a
+    // constant).  Per variable over all receive edges: all ADD, or all SET
+    // of one value, or written by a single component — then the writes of
+    // one broadcast's receivers give the same result in any order.
+    auto literal = [&](int x, double *v) {
+        if (x < 0 || n.exprs[x].op != SMC_E_LIT) return false;
+        *v = n.exprs[x].value;
+        return true;
+    };
+    auto is_var = [&](int x, int var) { return x >= 0 && n.exprs[x].op == SMC_E_VAR && n.exprs[x].var == var; };
+    enum { F_OTHER = 0, F_ADD = 1, F_SET = 2 };
+    auto form = [&](const smc_update &U, double *c) -> int {
+        const smc_expr_node *x = U.expr >= 0 ? &n.exprs[U.expr] : nullptr;
+        if (!x) return F_OTHER;
+        if (n.var_kind[U.var] == SMC_VAR_INT &&
+            ((x->op == SMC_E_ADD && is_var(x->a, U.var) && literal(x->b, c)) ||
+             (x->op == SMC_E_ADD && literal(x->a, c) && is_var(x->b, U.var)) ||
+             (x->op == SMC_E_SUB && is_var(x->a, U.var) && literal(x->b, c) && ((*c = -*c), true))) &&
+            *c == std::floor(*c) && std::fabs(*c) < 2147483648.0)
+            return F_ADD;
+        if (literal(U.expr, c)) return F_SET;
+        return F_OTHER;
+    };
+    struct VarUse { int add = 0, set = 0, other = 0, writer = -1; bool one_value = true; double value = 0.0; };
+    std::vector<VarUse> use(n.n_vars);
+    for (int c = 0; c < n.n_components; ++c)
+        for (int l = n.comp_first_loc[c]; l < n.comp_first_loc[c + 1]; ++l)
+            for (int k = 0; k < n.locations[l].edge_count; ++k) {
+                const smc_edge &E = n.edges[n.locations[l].edge_first + k];
+                if (E.channel >= 0) continue;
+                for (int u = 0; u < E.upd_count; ++u) {
+                    const smc_update &U = n.updates[E.upd_first + u];
+                    VarUse &vu = use[U.var];
+                    double cv = 0.0;
+                    const int f = form(U, &cv);
+                    if (f == F_ADD) ++vu.add;
+                    else if (f == F_SET) {
+                        if (vu.set && cv != vu.value) vu.one_value = false;
+                        vu.value = cv;
+                        ++vu.set;
+                    } else ++vu.other;
+                    vu.writer = (vu.writer == -1 || vu.writer == c) ? c : -2;
+                }
+            }
+    auto var_commutes = [&](int v) {
+        const VarUse &vu = use[v];
+        if (vu.other) return false;
+        if (vu.writer >= 0) return true;
+        return vu.set == 0 || (vu.add == 0 && vu.one_value);
+    };
+    std::vector<uint8_t> fast_edge(n.n_edges, 0);
+
     for (int e = 0; e < n.n_edges; ++e) {
         const smc_edge &E = n.edges[e];
         DevEdge d{};
@@ -211,17 +266,37 @@ ModelImage compile_model(const smc_network &n) {
         if (ng > 65535) throw std::runtime_error("limit: more than 65535 guards on an edge");
         d.guard_n = (uint16_t)ng;
         d.upd_first = (uint32_t)img.upds.size();
+        bool commute = E.channel < 0 && SMC_REF_KIND(d.weight) == SMC_REF_CONST;
+        std::vector<int> seen;
         for (int k = 0; k < E.upd_count; ++k) {
             const smc_update &U = n.updates[E.upd_first + k];
             if (U.var > 65535) throw std::runtime_error("limit: assigned variable index above 65535");
             DevUpd u{};
             u.var = (uint16_t)U.var;
             u.expr = lw.ref(U.expr, 0.0);
+            double cv = 0.0;
+            const int f = E.channel < 0 ? form(U, &cv) : F_OTHER;
+            if (f != F_OTHER && var_commutes(U.var)) {
+                u.kind = f == F_ADD ? SMC_U_ADD : SMC_U_SET;
+                u.value = cv;
+            } else {
+                commute = false;
+            }
+            if (std::find(seen.begin(), seen.end(), U.var) != seen.end()) commute = false;
+            seen.push_back(U.var);
             img.upds.push_back(u);
         }
+        fast_edge[e] = commute ? 1 : 0;
         if (E.upd_count > 65535) throw std::runtime_error("limit: more than 65535 updates on an edge");
         d.upd_n = (uint16_t)E.upd_count;
         img.edges.push_back(d);
+    }
+    for (int l = 0; l < n.n_locations; ++l) {
+        const smc_location &L = n.locations[l];
+        uint32_t bits = 0;
+        for (int k = 0; k < L.edge_count && k < 32; ++k)
+            if (fast_edge[L.edge_first + k]) bits |= 1u << k;
+        img.locs[l].recv_fast = bits;
     }
     // keep every table non-empty so device pointers are valid
     if (img.invs.empty()) img.invs.push_back(DevInv{});

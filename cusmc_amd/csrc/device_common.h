@@ -178,6 +178,109 @@ __device__ __forceinline__ int fire_edge(int e, double *vars, const DevModel &m,
 }
 
 template <int C, class T>
+__device__ __forceinline__ T slot_get(const T (&a)[C], int j);
+template <int C, class T>
+__device__ __forceinline__ void slot_set(T (&a)[C], int j, T v);
+
+// Receivers of one broadcast fired in parallel (SyncInSerial
+// simulation.cu:378-404 fires them one by one in component order).  Valid
+// when every enabled receive edge of the segment commutes (DevLoc.recv_fast):
+// constant weights make each receiver's pick independent of the others'
+// updates, and integer additions and private constants give the same
+// variables in any order.  The query variable's running max/min needs the
+// values in component order: they are v0 plus the prefix sums of the
+// additions, an exclusive scan over the segment (blocked layout: lane order
+// then slot order is component order).  TRACE mixes the fired edges into
+// the digest in component order.  Returns false, having changed nothing,
+// when some enabled edge does not commute.
+template <int C>
+__device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int (&loc)[C], double *vars,
+                                                    const DevModel &m, LaneQuery &q, bool &goal, uint64_t seed,
+                                                    uint64_t run, uint32_t step, int sl, int S, int seg_base,
+                                                    uint64_t seg_mask, uint32_t &trans, bool trace,
+                                                    uint64_t &digest) {
+    bool slow = false;
+#pragma unroll
+    for (int j = 0; j < C; ++j)
+        if (mk[j] && (mk[j] & ~(uint64_t)m.locs[loc[j]].recv_fast)) slow = true;
+    if (__ballot(slow) & seg_mask) return false;
+
+    int ej[C];
+    int fired = 0;
+    bool any_fired = false;
+    double qsum = 0.0, qhi = -DBL_MAX, qlo = DBL_MAX;  // prefix sums of this lane's query additions
+    bool qany = false;
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        ej[j] = -1;
+        if (mk[j]) {
+            const int first = (int)m.locs[loc[j]].edge_first;
+            const int e = first + pick_edge(mk[j], first, seed, run, step, SMC_P_RECEIVER,
+                                            (uint32_t)(sl * C + j), vars, m);
+            ej[j] = e;
+            ++fired;
+            any_fired = true;
+            const DevEdge &E = m.edges[e];
+            for (int k = 0; k < E.upd_n; ++k) {
+                const DevUpd U = m.upds[E.upd_first + k];
+                if ((int)U.var != q.qvar) continue;
+                if (U.kind == SMC_U_ADD) {
+                    qsum += U.value;
+                    qhi = fmax(qhi, qsum);
+                    qlo = fmin(qlo, qsum);
+                    qany = true;
+                } else {
+                    q.qmax = fmax(q.qmax, U.value);
+                    q.qmin = fmin(q.qmin, U.value);
+                }
+            }
+        }
+    }
+    if (q.qvar >= 0 && (__ballot(qany) & seg_mask)) {
+        double incl = qsum;
+        for (int k = 1; k < S; k <<= 1) {
+            const double y = __shfl_up(incl, k, S);
+            if (sl >= k) incl += y;
+        }
+        const double v0 = vars[q.qvar] + (incl - qsum);  // value before this lane's additions
+        if (qany) {
+            q.qmax = fmax(q.qmax, v0 + qhi);
+            q.qmin = fmin(q.qmin, v0 + qlo);
+        }
+    }
+    wave_sync();  // every lane has read the variables before they change
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        if (ej[j] >= 0) {
+            const DevEdge &E = m.edges[ej[j]];
+            for (int k = 0; k < E.upd_n; ++k) {
+                const DevUpd U = m.upds[E.upd_first + k];
+                if (U.kind == SMC_U_ADD) atomicAdd(&vars[U.var], U.value);
+                else vars[U.var] = U.value;
+            }
+            if (E.dst_goal) goal = true;
+            loc[j] = E.dst;
+        }
+    }
+    wave_sync();
+    for (int k = 1; k < S; k <<= 1) fired += __shfl_xor(fired, k);
+    trans += (uint32_t)fired;
+    if (trace) {
+        for (uint64_t lanes_with = __ballot(any_fired) & seg_mask; lanes_with; lanes_with &= lanes_with - 1) {
+            const int L = lowest_lane(lanes_with);
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                const int e = __shfl(ej[j], L);
+                if (e >= 0)
+                    digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)((L - seg_base) * C + j) << 20) |
+                                                 (uint64_t)e);
+            }
+        }
+    }
+    return true;
+}
+
+template <int C, class T>
 __device__ __forceinline__ T slot_get(const T (&a)[C], int j) {
     T r = a[0];
 #pragma unroll

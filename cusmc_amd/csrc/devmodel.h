@@ -61,7 +61,7 @@ struct DevLoc {             // 32 bytes, one pair of 16-byte loads
     uint16_t edge_n;
     uint8_t rate_const;
     uint8_t pad0;
-    uint32_t pad1;
+    uint32_t recv_fast;     // bit i: edge i is a commuting receive edge (see DevUpd)
 };
 
 struct DevInv {             // x <= bound  (strict: x < bound), 16 bytes
@@ -93,10 +93,18 @@ struct DevGuard {           // variables[var] op rhs
     uint32_t rhs;
 };
 
-struct DevUpd {
+// Update kinds: a commuting receive edge has only SMC_U_ADD updates
+// (integer variable += integer constant) and SMC_U_SET updates (a variable
+// written by one component only := constant), so the receivers of one
+// broadcast can fire in any order with the same result.
+enum { SMC_U_EXPR = 0, SMC_U_ADD = 1, SMC_U_SET = 2 };
+
+struct DevUpd {             // 16 bytes
     uint16_t var;
-    uint16_t pad;
+    uint8_t kind;           // SMC_U_*
+    uint8_t pad;
     uint32_t expr;
+    double value;           // the added constant (ADD) or the value (SET)
 };
 
 // Everything the run kernel needs to know about one model; passed by value.

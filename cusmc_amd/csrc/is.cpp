@@ -89,7 +89,7 @@ struct smc_is {
     smc_model *m = nullptr;
     smc_query q{};
     smc_is_config cfg{};
-    int S = 1, C = 1, R = 64, stride = 1;
+    int S = 1, C = 1, R = 64, stride = 1, occ = 0;
     size_t lds = 0;
     int n_locs = 0, n_vars = 0, n_comp = 0, F = 0, fpad = 0, H = 128, Lpad = 0;
     int n_groups = 0, Gpad = 0;  // tilt groups: the MLP's outputs
@@ -116,7 +116,7 @@ struct smc_is {
     DevBuf<float> theta, thg;
     // recorded (run, macro-step) pairs
     DevBuf<uint16_t> rX;
-    DevBuf<float> rA, rB, rw;
+    DevBuf<float> rA, rB, rw, rAg, rBg;  // per location, and summed per tilt group
     size_t rec_cap = 0;
     // training activations of one chunk of pairs
     DevBuf<uint16_t> th1, th2, Gb, dh2b, dh1b, xin;
@@ -214,23 +214,30 @@ struct smc_is {
         g3.M = rows; g3.N = Gpad; g3.K = H; g3.bias = w.p + o_b3; g3.clamp_max = cfg.tilt_max;
         g3.col_mask = gtiltable.p;
         hip_check(smc::gemm(g3, false, false, stream()), "MLP layer 3");
-        hip_check(smc::expand_groups(og, Gpad, group.p, n_locs, Lpad, rows, out, stream()), "expand tilts");
+        if (out) hip_check(smc::expand_groups(og, Gpad, group.p, n_locs, Lpad, rows, out, stream()), "expand tilts");
     }
 
+    // K splits for the weight gradients (K = rows): about 512 workgroups,
+    // each layer at least 1024 rows deep, so the fold stays small
     static int splits_for(int M, int N, int K) {
         const int tiles = (M / 64) * (N / 64);
-        int s = std::min(K / 256, 2048 / std::max(tiles, 1));
-        return std::max(1, std::min(s, 256));
+        int s = std::min(K / 1024, 512 / std::max(tiles, 1));
+        return std::max(1, std::min(s, 128));
     }
 
     // Accumulate into g the parameter gradient of sum(G * theta) for rows of
-    // a chunk: tth holds G per location (fp32), th1/th2 the activations.  The
-    // gradient of each group output is the sum over its locations (tthg and
-    // its bf16 copy Gb).
+    // a chunk, G given per location in tth: the gradient of a group output is
+    // the sum over its locations.
     void backward(const uint16_t *Xc, int rows) {
+        hip_check(smc::group_sum(tth.p, Lpad, member_first.p, members.p, n_groups, Gpad, rows, tthg.p, Gb.p,
+                                 stream()), "group gradient");
+        backward_groups(Xc, rows);
+    }
+
+    // The same from the group outputs' gradient: tthg (fp32) and Gb (bf16);
+    // th1/th2 hold the activations.
+    void backward_groups(const uint16_t *Xc, int rows) {
         hipStream_t s = stream();
-        hip_check(smc::group_sum(tth.p, Lpad, member_first.p, members.p, n_groups, Gpad, rows, tthg.p, Gb.p, s),
-                  "group gradient");
         const int csplit = std::max(1, std::min(1024, rows / 128));
         GemmArgs a{};
         a.A = th2.p; a.lda = H; a.B = Gb.p; a.ldb = Gpad; a.C = g.p + o_w3; a.ldc = Gpad;
@@ -259,7 +266,7 @@ struct smc_is {
 
     struct Batch {
         Tilt tilt = Tilt::NONE;
-        bool lockstep = true, early_stop = true, record = false;
+        bool lockstep = true, early_stop = true, record = false, trace = false;
         const uint64_t *d_runs = nullptr;  // run of each row, or run_begin + row
         uint64_t run_begin = 0, seed = 0;
         int n_valid = 0;
@@ -276,6 +283,7 @@ struct smc_is {
         P.feat = feats ? X[0].p : nullptr;
         hip_check(smc::is_init_rows(P, b.d_runs, b.run_begin, b.n_valid, feat0.p, s), "init rows");
         P.early_stop = b.early_stop ? 1 : 0;
+        P.trace = b.trace ? 1 : 0;
         P.stop_at_transition = b.lockstep ? 1 : 0;
         if (b.tilt == Tilt::FIXED) { P.theta = theta_fixed.p; P.theta_stride = 0; }
         if (b.record) {
@@ -296,7 +304,7 @@ struct smc_is {
                 P.rec_pos = pos.p;
             }
             hip_check(hipMemsetAsync(counters.p, 0, sizeof(unsigned int), s), "clear active count");
-            hip_check(smc::is_launch_advance(P, C, lds, s), "advance kernel launch");
+            hip_check(smc::is_launch_advance(P, C, occ, lds, s), "advance kernel launch");
             ++launches;
             cur ^= 1;
             if (!b.lockstep) break;
@@ -314,12 +322,14 @@ struct smc_is {
     void ensure_records() {
         if (rec_cap) return;
         // pairs of (features, wins, ln2 dt r) within 8 GiB
-        const size_t per_pair = (size_t)fpad * 2 + (size_t)Lpad * 8 + 4;
+        const size_t per_pair = (size_t)fpad * 2 + (size_t)Lpad * 8 + (size_t)Gpad * 8 + 4;
         rec_cap = std::min<size_t>(((size_t)8 << 30) / per_pair, (size_t)1 << 23) / 64 * 64;
         rX.ensure(rec_cap * fpad, "hipMalloc records");
         rA.ensure(rec_cap * Lpad, "hipMalloc records");
         rB.ensure(rec_cap * Lpad, "hipMalloc records");
         rw.ensure(rec_cap, "hipMalloc records");
+        rAg.ensure(rec_cap * Gpad, "hipMalloc records");
+        rBg.ensure(rec_cap * Gpad, "hipMalloc records");
         ensure_train(kTrainChunk);
     }
 
@@ -343,6 +353,14 @@ struct smc_is {
             hip_check(hipMemsetAsync(rw.p + pairs, 0, z * 4, st), "clear records");
         }
         rec_pairs = padded;
+        // statistics per tilt group (theta is shared inside a group), once
+        for (size_t off = 0; off < padded; off += kTrainChunk) {
+            const int rows_c = (int)std::min<size_t>(kTrainChunk, padded - off);
+            hip_check(smc::group_sum(rA.p + off * Lpad, Lpad, member_first.p, members.p, n_groups, Gpad, rows_c,
+                                     rAg.p + off * Gpad, nullptr, st), "group wins");
+            hip_check(smc::group_sum(rB.p + off * Lpad, Lpad, member_first.p, members.p, n_groups, Gpad, rows_c,
+                                     rBg.p + off * Gpad, nullptr, st), "group rates");
+        }
         return pairs;
     }
 
@@ -353,10 +371,11 @@ struct smc_is {
         for (size_t off = 0; off < rec_pairs; off += kTrainChunk) {
             const int rows_c = (int)std::min<size_t>(kTrainChunk, rec_pairs - off);
             const uint16_t *Xc = rX.p + off * fpad;
-            forward(Xc, rows_c, th1.p, th2.p, tthg.p, tth.p);
-            hip_check(smc::is_ce_grad(tth.p, rA.p + off * Lpad, rB.p + off * Lpad, rw.p + off, tiltable.p,
-                                      cfg.tilt_max, rows_c, Lpad, tth.p, st), "output gradient");
-            backward(Xc, rows_c);
+            forward(Xc, rows_c, th1.p, th2.p, tthg.p, nullptr);
+            hip_check(smc::is_ce_grad(tthg.p, rAg.p + off * Gpad, rBg.p + off * Gpad, rw.p + off, gtiltable.p,
+                                      cfg.tilt_max, rows_c, Gpad, tthg.p, st), "output gradient");
+            hip_check(smc::to_bf16(tthg.p, Gb.p, (size_t)rows_c * Gpad, st), "output gradient to bf16");
+            backward_groups(Xc, rows_c);
         }
     }
 
@@ -403,8 +422,15 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         hip_check(hipSetDevice(m->device), "hipSetDevice");
 
         // launch shape of the advance kernel
+        // lanes per run: the model's tuned packing (smc_autotune) when there
+        // is one, else the widest
         const int P = img.n_leaves;
         s->S = P < 64 ? P : 64;
+        if (m->has_tuned && m->tuned.lanes_per_run > 0 && m->tuned.lanes_per_run <= s->S &&
+            P / m->tuned.lanes_per_run <= 16) {
+            s->S = m->tuned.lanes_per_run;
+            s->occ = m->tuned.occupancy == 2 ? 2 : 0;
+        }
         s->C = P / s->S;
         if (s->C > 16) throw std::runtime_error("limit: more than 1024 components");
         s->R = 64 / s->S;
@@ -658,6 +684,7 @@ int smc_is_sample(smc_is *s, const float *theta, int use_mlp, uint64_t seed, uin
         b.tilt = theta ? Tilt::FIXED : (use_mlp ? Tilt::MLP : Tilt::NONE);
         b.lockstep = lockstep != 0;
         b.early_stop = early_stop != 0;
+        b.trace = out && out->digest;
         b.seed = seed;
         if (theta) {
             std::vector<float> th(s->Lpad, 0.f);

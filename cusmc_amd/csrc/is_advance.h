@@ -1,0 +1,512 @@
+// is_advance.h — the tilted advance kernel of importance sampling on gfx950
+// (compiled once per slot count and register policy by is_variant.hip) and
+// the bf16 helpers of the MLP kernels.
+//
+// No counterpart in the reference (cuSMC estimates by plain Monte Carlo,
+// statistics.cuh:126-163).  A proposal multiplies the rate of each
+// exponential component by 2^theta, with theta per location given by a small
+// MLP of the run's state; each run carries log2 of its likelihood ratio in
+// FP64, so that hit * 2^log2lr is an unbiased estimate of the probability.
+//
+// Runs advance in lock step by macro-steps: one launch of is_advance_kernel
+// moves every active run to its next transition (the race, clocks and the
+// winner's edge exactly as the run kernel does them, with tilted rates), then
+// writes the run's features for the MLP, whose forward pass (MFMA GEMMs
+// below) produces the tilts of the next macro-step.
+//
+// Cross-entropy training replays the elite runs once with the tilts that
+// generated them and records, per (run, macro-step), the features at the
+// macro-step's start and two statistics per location L of the run's
+// components: A_L, the number of races won by the component in L, and
+// B_L = ln2 * dt * r_L, its base rate times the macro-step's duration.  The
+// log-density of the macro-step under tilts theta is then
+//   ln q = sum_L ln2 (A_L theta_L - B_L 2^theta_L) + terms free of theta,
+// so every training epoch is a pass of GEMMs over the recorded pairs.
+//
+// Likelihood ratio of one race step (exponential race with base rates r_c,
+// tilted rates r'_c = r_c 2^theta_c, sums R and R', delay d, delays drawn
+// as -log2(u)/rate so the effective rates carry a factor ln 2):
+//   LR = (r_w / r'_w) exp(-(R - R') ln2 d)   =>   log2 LR = -theta_w + (R' - R) d
+// where the first term is present only when an exponential component won.
+// Uniform and zero delays are not tilted and contribute nothing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "is_kernels.hpp"
+
+namespace {
+
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFF) ? 0x40 : 0));
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Pairwise tree sum over the segment, as in the run kernel (in-lane levels,
+// then the butterfly); every lane gets the total.
+template <int C>
+__device__ __forceinline__ double tree_total(const double (&leaf)[C], int S) {
+    double tr[2 * C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) tr[C + j] = leaf[j];
+#pragma unroll
+    for (int i = C - 1; i >= 1; --i) tr[i] = tr[2 * i] + tr[2 * i + 1];
+    double s = (C == 1) ? leaf[0] : tr[1];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        if ((1 << k) < S) s = s + seg_xchg(s, k);
+    return s;
+}
+
+// Features of a run: one-hot of each component's location, scaled
+// variables, time / bound.  Each entry is written by exactly one lane.
+template <int C>
+__device__ __forceinline__ void write_features(const IsParams &P, uint16_t *f, const int (&loc)[C], const double *vars,
+                                               double t, int sl, int S, int n) {
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        const int c = sl * C + j;
+        if (c < n) {
+            for (int l = P.m.comp_first_loc[c]; l < P.m.comp_first_loc[c + 1]; ++l)
+                f[l] = loc[j] == l ? (uint16_t)0x3F80 : (uint16_t)0;
+        }
+    }
+    const int base = P.m.n_locs;
+    for (int v = sl; v < P.m.n_vars; v += S) f[base + v] = f32_to_bf16((float)(vars[v] * P.var_scale[v]));
+    if (sl == 0) {
+        f[base + P.m.n_vars] = f32_to_bf16((float)(t / P.bound));
+        for (int k = base + P.m.n_vars + 1; k < P.fpad; ++k) f[k] = 0;
+    }
+}
+
+// Whether a var-probability query is already decided: the segment-wide
+// running max (min) has crossed the threshold.  Every lane tracks the
+// updates it executed, so the test reduces over the segment.
+__device__ __forceinline__ bool settled(const LaneQuery &q, const IsParams &P, int S) {
+    double v = P.is_max ? q.qmax : q.qmin;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        if ((1 << k) < S) v = P.is_max ? fmax(v, seg_xchg(v, k)) : fmin(v, seg_xchg(v, k));
+    return P.is_max ? v > P.threshold : v < P.threshold;
+}
+
+// ------------------------------------------------------------ advance
+
+template <int C>
+__device__ __forceinline__ void is_advance_body(const IsParams &P) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const int S = P.lanes;
+    const int sl = lane & (S - 1);
+    const int seg_base = lane - sl;
+    const uint64_t seg_mask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << seg_base);
+    const int row = blockIdx.x * (64 / S) + lane / S;
+    if (row >= P.n_rows) return;  // whole segments only
+    double *vars = lds + (size_t)(lane / S) * P.var_stride;
+    const DevModel &m = P.m;
+    const IsRows &st = P.st;
+    const int n = m.n_comp;
+    const double INF = __builtin_huge_val();
+    const double LN2 = 0.69314718055994530942;
+
+    uint8_t stat = st.status[row];
+    if (!(stat & IS_ACTIVE)) {
+        if (P.feat) {
+            uint16_t *f = P.feat + (size_t)row * P.fpad;
+            for (int k = sl; k < P.fpad; k += S) f[k] = 0;
+        }
+        return;
+    }
+
+    // load the run
+    const uint64_t run = st.run[row];
+    double t = st.t[row];
+    const double t_start = t;
+    double log2lr = st.log2lr[row];
+    uint32_t step = st.step[row], trans = st.trans[row];
+    int chan = st.chan[row], sender = st.sender[row];
+    bool goal = (stat & IS_GOAL) != 0, goal_local = false;
+    bool mid = (stat & IS_MID) != 0;
+    LaneQuery q{P.qvar, st.qmax[row], st.qmin[row]};
+    uint64_t digest = st.digest[row];
+    for (int v = sl; v < m.n_vars; v += S) vars[v] = st.vars[(size_t)row * m.n_vars + v];
+    int loc[C], loc0[C];
+    double r0[C];
+    uint32_t wins[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        const int c = sl * C + j;
+        loc[j] = c < n ? (int)st.loc[(size_t)row * n + c] : -1;
+        loc0[j] = loc[j];
+        r0[j] = 0.0;
+        wins[j] = 0;
+    }
+    wave_sync();
+    const float *theta = P.theta ? P.theta + (size_t)row * P.theta_stride : nullptr;
+    const int rec = P.rec_pos ? P.rec_pos[row] : -1;
+    if (rec >= 0) {
+        write_features<C>(P, P.rec_X + (size_t)rec * P.fpad, loc, vars, t, sl, S, n);
+        if (sl == 0) P.rec_w[rec] = P.weight[row];
+    }
+    bool first_race = true;
+    bool hit = false, end = false, fired_any = false;
+
+    for (;;) {
+        if (!mid) {
+            // loop test (simulation.cu:835-849), with the early stop of a
+            // var-probability query once its outcome is settled
+            if (!(t < P.bound)) end = true;
+            else if (P.reach && goal) { end = true; hit = true; }
+            else if (P.early_stop && !P.reach && settled(q, P, S)) { end = true; hit = true; }
+            else if (step >= P.max_steps) end = true;
+            if (end) break;
+            if (fired_any && P.stop_at_transition) {
+                // process the pending broadcast now so the features show it;
+                // the next launch resumes at the race (mid-iteration)
+                if (chan > 0) {
+                    uint64_t mk[C];
+                    bool any = false;
+#pragma unroll
+                    for (int j = 0; j < C; ++j) {
+                        const int c = sl * C + j;
+                        mk[j] = (c < n && c != sender) ? enabled_mask(loc[j], chan, vars, m) : 0ull;
+                        any |= mk[j] != 0;
+                    }
+                    int dummy = 0;
+                    const bool fast = fire_receivers_fast<C>(mk, loc, vars, m, q, goal_local, P.seed, run, step, sl,
+                                                             S, seg_base, seg_mask, trans, P.trace != 0, digest);
+                    for (uint64_t lw = fast ? 0ull : (__ballot(any) & seg_mask); lw; lw &= lw - 1) {
+                        const int L = lowest_lane(lw);
+#pragma clang loop unroll(disable)
+                        for (int j = 0; j < C; ++j) {
+                            if (!((__ballot(slot_get<C>(mk, j) != 0) >> L) & 1ull)) continue;
+                            const int c = (L - seg_base) * C + j;
+                            int e_fired = -1;
+                            if (lane == L) {
+                                const int lj = slot_get<C>(loc, j);
+                                const int first = (int)m.locs[lj].edge_first;
+                                const int i = pick_edge(slot_get<C>(mk, j), first, P.seed, run, step, SMC_P_RECEIVER,
+                                                        (uint32_t)c, vars, m);
+                                e_fired = first + i;
+                                slot_set<C>(loc, j, fire_edge(e_fired, vars, m, q, goal_local, dummy));
+                            }
+                            wave_sync();
+                            ++trans;
+                            if (P.trace) {
+                                const int e = __shfl(e_fired, L);
+                                digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)c << 20) | (uint64_t)e);
+                            }
+                        }
+                    }
+                    chan = 0;
+                    sender = -1;
+                    goal = (__ballot(goal_local) & seg_mask) != 0;
+                }
+                mid = true;
+                break;
+            }
+        }
+        double d = 0.0;
+        int w = -1;
+#pragma clang loop unroll(disable)
+        for (int phase = mid ? 1 : 0; phase < 2; ++phase) {
+            if (phase == 0 && chan <= 0) continue;
+            bool exp_won = false;
+            if (phase == 1) {
+                double leaf[C], leaft[C], Mb[C];
+                int zero_j = C;
+                uint32_t bmask = 0;
+#pragma unroll
+                for (int j = 0; j < C; ++j) {
+                    leaf[j] = 0.0; leaft[j] = 0.0; Mb[j] = 0.0;
+                    const int c = sl * C + j;
+                    if (c < n) {
+                        const DevLoc L = m.locs[loc[j]];
+                        if (L.klass == SMC_K_ZERO) {
+                            if (zero_j == C) zero_j = j;
+                        } else {
+                            bool bounded = false;
+                            double M = DBL_MAX;
+                            for (int i = 0; i < L.n_inv; ++i) {
+                                const DevInv iv = m.invs[L.inv_first + i];
+                                const double bound = iv.bound_const ? iv.bound_value : eval_ref(iv.bound, vars, m);
+                                double ttb = bound - vars[iv.var];
+                                if (iv.strict) ttb -= 1e-6;
+                                if (ttb >= 0.0) { M = ttb < M ? ttb : M; bounded = true; }
+                            }
+                            if (bounded) {
+                                if (M == 0.0) { if (zero_j == C) zero_j = j; }
+                                else { Mb[j] = M; bmask |= 1u << j; }
+                            } else {
+                                const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
+                                leaf[j] = r > 0.0 ? r : 0.0;
+                                leaft[j] = leaf[j];
+                                if (theta && leaf[j] > 0.0 && P.tiltable[loc[j]])
+                                    leaft[j] = leaf[j] * smc_exp2((double)theta[loc[j]]);
+                            }
+                        }
+                    }
+                }
+                if (first_race) {
+#pragma unroll
+                    for (int j = 0; j < C; ++j) r0[j] = leaf[j];
+                    first_race = false;
+                }
+                const uint64_t zb = __ballot(zero_j < C) & seg_mask;
+                if (zb) {
+                    const int Lz = lowest_lane(zb);
+                    w = (Lz - seg_base) * C + __shfl(zero_j, Lz);
+                    d = 0.0;
+                } else {
+                    double umin = INF;
+                    int unif_j = -1;
+                    for (uint32_t b = bmask; b; b &= b - 1) {
+                        const int j = __ffs(b) - 1;
+                        const smc_u32x4 rw = smc_draw(P.seed, run, step, SMC_P_UNIFORM, (uint32_t)(sl * C + j));
+                        const double du = slot_get<C>(Mb, j) * smc_u01(rw.v[0], rw.v[1]);
+                        if (du < umin) { umin = du; unif_j = j; }
+                    }
+                    // tilted race: the run kernel's tree over the tilted rates
+                    double tr[2 * C];
+#pragma unroll
+                    for (int j = 0; j < C; ++j) tr[C + j] = leaft[j];
+#pragma unroll
+                    for (int i = C - 1; i >= 1; --i) tr[i] = tr[2 * i] + tr[2 * i + 1];
+                    double s = (C == 1) ? leaft[0] : tr[1];
+                    double own[6], sib[6];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        own[k] = s;
+                        sib[k] = 0.0;
+                        if ((1 << k) < S) { sib[k] = seg_xchg(s, k); s = s + sib[k]; }
+                    }
+                    const double Rt = s;
+                    const double R = theta ? tree_total<C>(leaf, S) : Rt;
+                    double dexp = INF;
+                    int wexp = 0x7FFFFFFF;
+                    if (Rt > 0.0) {
+                        const smc_u32x4 rw = smc_draw(P.seed, run, step, SMC_P_RACE, 0);
+                        dexp = -smc_log2(smc_u01(rw.v[0], rw.v[1])) / Rt;
+                        double v = smc_u01(rw.v[2], rw.v[3]) * Rt;
+                        int b = 0;
+#pragma unroll
+                        for (int k = 5; k >= 0; --k) {
+                            if ((1 << k) < S) {
+                                const bool in_left = ((sl >> k) & 1) == 0;
+                                const double left = in_left ? own[k] : sib[k];
+                                const double right = in_left ? sib[k] : own[k];
+                                if (!(v < left || right == 0.0)) { v = v - left; b += 1 << k; }
+                            }
+                        }
+                        int node = 1;
+#pragma unroll
+                        for (int lv = 0; (1 << lv) < C; ++lv) {
+                            double left = 0.0, right = 0.0;
+#pragma unroll
+                            for (int cand = (1 << lv); cand < (2 << lv); ++cand)
+                                if (node == cand) { left = tr[2 * cand]; right = tr[2 * cand + 1]; }
+                            if (v < left || right == 0.0) node = 2 * node;
+                            else { v = v - left; node = 2 * node + 1; }
+                        }
+                        const int Lw = lowest_lane(__ballot(sl == b) & seg_mask);
+                        wexp = (Lw - seg_base) * C + __shfl(node - C, Lw);
+                    }
+                    double gu = umin;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k)
+                        if ((1 << k) < S) gu = fmin(gu, seg_xchg(gu, k));
+                    int wu = 0x7FFFFFFF;
+                    if (gu < INF) {
+                        const int Lu = lowest_lane(__ballot(unif_j >= 0 && umin == gu) & seg_mask);
+                        wu = (Lu - seg_base) * C + __shfl(unif_j, Lu);
+                    }
+                    if (gu < dexp || (gu == dexp && wu < wexp)) { d = gu; w = wu; }
+                    else { d = dexp; w = wexp; exp_won = true; }
+                    if (theta && d < INF) {
+                        // log2 LR of this step: -theta_w + (R' - R) d
+                        double th_w = 0.0;
+                        if (exp_won) {
+                            const int wl_lane = seg_base + w / C;
+                            const int wj = w % C;
+                            double tw = 0.0;
+                            if (lane == wl_lane) {
+                                const int lw = slot_get<C>(loc, wj);
+                                if (P.tiltable[lw]) tw = (double)theta[lw];
+                            }
+                            th_w = __shfl(tw, wl_lane);
+                        }
+                        log2lr = log2lr - th_w + (Rt - R) * d;
+                    }
+                    if (exp_won && rec >= 0) {
+                        const int wj = w % C;
+                        if (lane == seg_base + w / C) {
+#pragma unroll
+                            for (int j = 0; j < C; ++j) wins[j] += (j == wj) ? 1u : 0u;
+                        }
+                    }
+                }
+                if (!(d < INF)) { t = INF; end = true; break; }
+                for (int i = sl; i < m.n_clocks; i += S) vars[m.clocks[i]] += d;
+                wave_sync();
+                // the winner's transition, once, on the lane holding it
+                const int wl = seg_base + w / C;
+                int e_fired = -1, new_chan = 0;
+                if (lane == wl) {
+                    const int wj = w % C;
+                    const int lj = slot_get<C>(loc, wj);
+                    const uint64_t mkw = enabled_mask(lj, 0, vars, m);
+                    if (mkw) {
+                        const int first = (int)m.locs[lj].edge_first;
+                        const int i = pick_edge(mkw, first, P.seed, run, step, SMC_P_WINNER, (uint32_t)w, vars, m);
+                        e_fired = first + i;
+                        slot_set<C>(loc, wj, fire_edge(e_fired, vars, m, q, goal_local, new_chan));
+                    }
+                }
+                wave_sync();
+                e_fired = __shfl(e_fired, wl);
+                if (e_fired >= 0) {
+                    ++trans;
+                    if (P.trace)
+                        digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)w << 20) | (uint64_t)e_fired);
+                    chan = __shfl(new_chan, wl);
+                    sender = chan > 0 ? w : -1;
+                    fired_any = true;
+                }
+                goal = (__ballot(goal_local) & seg_mask) != 0;
+                break;
+            }
+            // phase 0: pending broadcast receivers
+            uint64_t mk[C];
+            bool any = false;
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                const int c = sl * C + j;
+                mk[j] = (c < n && c != sender) ? enabled_mask(loc[j], chan, vars, m) : 0ull;
+                any |= mk[j] != 0;
+            }
+            int dummy_chan = 0;
+            const bool fast = fire_receivers_fast<C>(mk, loc, vars, m, q, goal_local, P.seed, run, step, sl, S,
+                                                     seg_base, seg_mask, trans, P.trace != 0, digest);
+            for (uint64_t lanes_with = fast ? 0ull : (__ballot(any) & seg_mask); lanes_with;
+                 lanes_with &= lanes_with - 1) {
+                const int L = lowest_lane(lanes_with);
+#pragma clang loop unroll(disable)
+                for (int j = 0; j < C; ++j) {
+                    if (!((__ballot(slot_get<C>(mk, j) != 0) >> L) & 1ull)) continue;
+                    const int c = (L - seg_base) * C + j;
+                    int e_fired = -1;
+                    if (lane == L) {
+                        const int lj = slot_get<C>(loc, j);
+                        const int first = (int)m.locs[lj].edge_first;
+                        const int i = pick_edge(slot_get<C>(mk, j), first, P.seed, run, step, SMC_P_RECEIVER,
+                                                (uint32_t)c, vars, m);
+                        e_fired = first + i;
+                        slot_set<C>(loc, j, fire_edge(e_fired, vars, m, q, goal_local, dummy_chan));
+                    }
+                    wave_sync();
+                    ++trans;
+                    if (P.trace) {
+                        const int e = __shfl(e_fired, L);
+                        digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)c << 20) | (uint64_t)e);
+                    }
+                }
+            }
+            chan = 0;
+            sender = -1;
+            goal = (__ballot(goal_local) & seg_mask) != 0;
+        }
+        mid = false;
+        if (end) break;
+        t = t + d;
+        ++step;
+    }
+
+    // store the run
+    const bool still = !end;
+    if (end) {
+        double qmax = q.qmax, qmin = q.qmin;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if ((1 << k) < S) { qmax = fmax(qmax, seg_xchg(qmax, k)); qmin = fmin(qmin, seg_xchg(qmin, k)); }
+        q.qmax = qmax; q.qmin = qmin;
+        if (!hit && !P.reach) hit = P.is_max ? (qmax > P.threshold) : (qmin < P.threshold);
+        if (P.trace) {
+            digest = smc_mix(digest, smc_digest_bits(t));
+            for (int v = 0; v < m.n_vars; ++v) digest = smc_mix(digest, smc_digest_bits(vars[v]));
+            for (int c = 0; c < n; ++c) {
+                const int lv = __shfl(slot_get<C>(loc, c % C), seg_base + c / C);
+                digest = smc_mix(digest, (uint64_t)(uint32_t)lv);
+            }
+        }
+    } else {
+        double qmax = q.qmax, qmin = q.qmin;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if ((1 << k) < S) { qmax = fmax(qmax, seg_xchg(qmax, k)); qmin = fmin(qmin, seg_xchg(qmin, k)); }
+        q.qmax = qmax; q.qmin = qmin;
+    }
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        const int c = sl * C + j;
+        if (c < n) st.loc[(size_t)row * n + c] = (uint16_t)loc[j];
+    }
+    for (int v = sl; v < m.n_vars; v += S) st.vars[(size_t)row * m.n_vars + v] = vars[v];
+    goal = (__ballot(goal_local) & seg_mask) != 0 || goal;
+    if (sl == 0) {
+        st.t[row] = t;
+        st.log2lr[row] = log2lr;
+        st.step[row] = step;
+        st.trans[row] = trans;
+        st.chan[row] = chan;
+        st.sender[row] = sender;
+        st.qmax[row] = q.qmax;
+        st.qmin[row] = q.qmin;
+        st.digest[row] = digest;
+        st.status[row] = (uint8_t)((still ? IS_ACTIVE : 0) | (hit ? IS_HIT : 0) | (goal ? IS_GOAL : 0) |
+                                   (still && mid ? IS_MID : 0));
+        if (still && P.active_count) atomicAdd(P.active_count, 1u);
+    }
+    if (P.feat) write_features<C>(P, P.feat + (size_t)row * P.fpad, loc, vars, t, sl, S, n);
+    if (rec >= 0) {
+        // statistics of the macro-step for the locations its components
+        // started in (rates are constant within a macro-step)
+        float *A = P.rec_A + (size_t)rec * P.lpad;
+        float *B = P.rec_B + (size_t)rec * P.lpad;
+        const double dt = (t < INF ? t : P.bound) - t_start;
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            const int c = sl * C + j;
+            if (c < n) {
+                for (int l = m.comp_first_loc[c]; l < m.comp_first_loc[c + 1]; ++l) {
+                    float a = 0.f, b = 0.f;
+                    if (l == loc0[j] && P.tiltable[l]) {
+                        a = (float)wins[j];
+                        b = (float)(LN2 * dt * r0[j]);
+                    }
+                    A[l] = a;
+                    B[l] = b;
+                }
+            }
+        }
+        if (sl == 0)
+            for (int k = m.n_locs; k < P.lpad; ++k) { A[k] = 0.f; B[k] = 0.f; }
+    }
+}
+
+// Registers unrestricted, or at most 256 (two waves per SIMD); see the run
+// kernel.
+template <int C>
+__global__ __launch_bounds__(64) void is_advance_kernel(IsParams P) {
+    is_advance_body<C>(P);
+}
+
+template <int C>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void is_advance_kernel_o2(IsParams P) {
+    is_advance_body<C>(P);
+}
+
+}  // namespace

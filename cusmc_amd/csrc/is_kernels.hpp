@@ -34,6 +34,7 @@ struct IsParams {
     uint32_t max_steps;
     int32_t stop_at_transition; // 1: lock-step macro-steps, 0: run to the end
     int32_t early_stop;         // end a var-probability run once it is decided
+    int32_t trace;              // keep the digest of every fired edge
     // proposal
     const float *theta;         // log2 tilts per location, or null (no tilt)
     int32_t theta_stride;       // row stride of theta (0: one row for every run)
@@ -68,7 +69,21 @@ struct GemmArgs {
 
 namespace smc {
 
-hipError_t is_launch_advance(const IsParams &p, int slots, size_t lds, hipStream_t s);
+// One build of the advance kernel (is_variant.hip).
+typedef hipError_t (*IsAdvanceLaunch)(const IsParams &p, int grid, size_t lds, hipStream_t s);
+IsAdvanceLaunch is_advance_c1_o0();
+IsAdvanceLaunch is_advance_c2_o0();
+IsAdvanceLaunch is_advance_c4_o0();
+IsAdvanceLaunch is_advance_c8_o0();
+IsAdvanceLaunch is_advance_c16_o0();
+IsAdvanceLaunch is_advance_c1_o2();
+IsAdvanceLaunch is_advance_c2_o2();
+IsAdvanceLaunch is_advance_c4_o2();
+IsAdvanceLaunch is_advance_c8_o2();
+IsAdvanceLaunch is_advance_c16_o2();
+
+// occ: 0 registers unrestricted, 2 at most 256 (two waves per SIMD)
+hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, hipStream_t s);
 // Reset rows [0, rows) to the initial state of runs runs[i] (or run_begin + i);
 // rows >= n_valid are inactive.  feat0 is the feature row of the initial state.
 hipError_t is_init_rows(const IsParams &p, const uint64_t *runs, uint64_t run_begin, int n_valid,
@@ -82,7 +97,8 @@ hipError_t is_ce_grad(const float *theta, const float *A, const float *B, const 
 // tl[r][l] = tg[r][group[l]] (0 for padding locations).
 hipError_t expand_groups(const float *tg, int gpad, const int32_t *group, int n_locs, int lpad, int rows, float *tl,
                          hipStream_t s);
-// gg[r][g] = sum of gl[r][l] over the members l of group g (fp32 and bf16).
+// gg[r][g] = sum of gl[r][l] over the members l of group g (fp32, and bf16
+// into ggb unless it is null).
 hipError_t group_sum(const float *gl, int lpad, const int32_t *member_first, const int32_t *members, int n_groups,
                      int gpad, int rows, float *gg, uint16_t *ggb, hipStream_t s);
 hipError_t gemm(const GemmArgs &g, bool ta, bool tb, hipStream_t s);

@@ -1,0 +1,28 @@
+// is_variant.hip — one build of the tilted advance kernel (is_advance.h):
+// SMC_C component slots per lane, register policy SMC_OCC (0 unrestricted,
+// 2 at most 256 registers).  The build compiles this file once per pair.
+#include "is_advance.h"
+
+#if !defined(SMC_C) || !defined(SMC_OCC)
+#error "SMC_C and SMC_OCC must be defined"
+#endif
+
+#define SMC_JOIN2(a, b, c, d) a##b##c##d
+#define SMC_JOIN(a, b, c, d) SMC_JOIN2(a, b, c, d)
+
+namespace {
+
+hipError_t launch(const IsParams &p, int grid, size_t lds, hipStream_t s) {
+#if SMC_OCC == 2
+    hipLaunchKernelGGL(is_advance_kernel_o2<SMC_C>, dim3(grid), dim3(64), lds, s, p);
+#else
+    hipLaunchKernelGGL(is_advance_kernel<SMC_C>, dim3(grid), dim3(64), lds, s, p);
+#endif
+    return hipGetLastError();
+}
+
+}  // namespace
+
+namespace smc {
+IsAdvanceLaunch SMC_JOIN(is_advance_c, SMC_C, _o, SMC_OCC)() { return &launch; }
+}  // namespace smc

@@ -52,8 +52,26 @@ struct ReducePartial {
 
 namespace smc {
 
-hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int waves, size_t lds_bytes, hipStream_t stream);
-hipError_t sim_occupancy(int slots, bool trace, size_t lds_bytes, int *blocks_per_cu);
+// One build of the run kernel (sim_variant.hip).
+struct SimVariant {
+    hipError_t (*launch)(const LaunchParams &lp, bool trace, int waves, size_t lds, hipStream_t s);
+    hipError_t (*occupancy)(bool trace, size_t lds, int *blocks_per_cu);
+};
+SimVariant sim_variant_c1_o0();
+SimVariant sim_variant_c2_o0();
+SimVariant sim_variant_c4_o0();
+SimVariant sim_variant_c8_o0();
+SimVariant sim_variant_c16_o0();
+SimVariant sim_variant_c1_o2();
+SimVariant sim_variant_c2_o2();
+SimVariant sim_variant_c4_o2();
+SimVariant sim_variant_c8_o2();
+SimVariant sim_variant_c16_o2();
+
+// occ: 0 registers unrestricted, 2 at most 256 (two waves per SIMD)
+hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int occ, int waves, size_t lds_bytes,
+                      hipStream_t stream);
+hipError_t sim_occupancy(int slots, bool trace, int occ, size_t lds_bytes, int *blocks_per_cu);
 hipError_t launch_reduce(const InstDesc *inst, int n_inst, uint32_t blocks_per_inst, ReducePartial *partials,
                          ReducePartial *acc, int accumulate, hipStream_t s);
 hipError_t launch_prep(unsigned long long *counter, uint64_t *round, int advance, hipStream_t s);

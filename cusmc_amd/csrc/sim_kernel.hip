@@ -1,38 +1,7 @@
-// sim_kernel.hip — the run kernel and the result reduction for gfx950.
+// sim_kernel.hip — the result reduction and the run kernel's launchers.
 //
-// Counterpart of cuSMC's simulation_kernel (simulation/simulation.cu:674-901)
-// and its device functions (compute_possible_delay :409-541,
-// find_minimum_delay :544-671, SyncInSerial :357-407, check_edge_enabled
-// :184-266, take_transition :27-181), and of the expression evaluator
-// (simulation/expressions.cu).  Same step semantics, different machine
-// mapping:
-//
-//   * cuSMC runs one block per run and one thread per component, with
-//     ~8 + log2(MC) __syncthreads per step and thread-0 serial sections.
-//     Here a run is a segment of S = 2^k lanes of one wave64 (64/S runs per
-//     wave), and each lane owns C = next_pow2(components)/S component slots
-//     (blocked: lane l holds components l*C .. l*C+C-1).  A workgroup is one
-//     wave, so no workgroup barrier is ever needed; lanes of a segment talk
-//     through cross-lane shuffles, ballots and the segment's LDS block.
-//   * Variables and clocks of each run live in LDS as doubles; clocks and
-//     global time accumulate in FP64.
-//   * The race: cuSMC re-samples every component's delay every step
-//     (simulation.cu:513-540) and takes the minimum.  Here the same law is
-//     sampled with far fewer draws: zero-delay components (branchpoint,
-//     urgent, committed, or a clock bound already reached) win first, lowest
-//     index first; otherwise the exponential components are one
-//     exponential race — minimum delay Exp(sum of rates) and winner chosen in
-//     proportion to its rate — and only components under a clock upper bound
-//     draw their own uniform delay.  The rate sum and the proportional
-//     choice use one fixed pairwise tree over the next power of two of the
-//     component count (in-lane levels, then a butterfly over the segment),
-//     so the result does not depend on the launch shape and a sequential
-//     host implementation reproduces it bit for bit.
-//   * Broadcast receivers compute their enabled sets in parallel and then
-//     fire one at a time in component order (SyncInSerial), each lane
-//     executing its own component's transition.
-//   * Runs are handed out by an atomic counter, so a segment that finishes a
-//     short run immediately starts the next one.
+// The run kernel itself (sim_body.h) is compiled once per slot count and
+// register policy (sim_variant.hip); this file dispatches to those builds.
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <stdint.h>
@@ -41,317 +10,6 @@
 #include "kernels.hpp"
 
 namespace {
-
-// ------------------------------------------------------------ run kernel
-
-
-// One wave at a time takes a chunk of runs of one instance from the launch's
-// queue, so the instance (its tables, query and outputs) is uniform across
-// the wave and lives in scalar registers; the wave's segments then pull the
-// chunk's runs one by one from a counter in LDS.
-template <int C, bool TRACE>
-__global__ __launch_bounds__(64) void sim_kernel(LaunchParams lp) {
-    extern __shared__ double lds[];
-    __shared__ unsigned int chunk_next;
-    const int lane = threadIdx.x;
-    const int S = lp.lanes;
-    const int sl = lane & (S - 1);
-    const int seg_base = lane - sl;
-    const uint64_t seg_mask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << seg_base);
-    double *vars = lds + (size_t)(lane / S) * lp.var_stride;
-    const double INF = __builtin_huge_val();
-    const uint64_t round = lp.round ? *lp.round : 0;
-
-  for (;;) {
-    unsigned long long g = 0;
-    if (lane == 0) g = atomicAdd(lp.counter, 1ull);
-    g = __shfl(g, 0);
-    if (g >= lp.total_chunks) break;
-    int lo = 0, hi = lp.n_inst - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (lp.inst[mid].chunk_first <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    const DevModel m = lp.inst[lo].m;
-    const SimParams p = lp.inst[lo].p;
-    const uint64_t chunk = g - lp.inst[lo].chunk_first;
-    const uint32_t r_first = (uint32_t)(chunk * lp.chunk_runs);
-    const uint32_t r_end = (uint32_t)((uint64_t)r_first + lp.chunk_runs < p.n_runs ? r_first + lp.chunk_runs : p.n_runs);
-    if (lane == 0) chunk_next = r_first;
-    wave_sync();
-    const int n = m.n_comp;
-
-    int loc[C];
-#pragma unroll
-    for (int j = 0; j < C; ++j) loc[j] = -1;
-
-    bool active = false, done = false;
-    uint64_t run = 0, ridx = 0;
-    double t = 0.0;
-    uint32_t step = 0, trans = 0;
-    int chan = 0, sender = -1;
-    bool goal = false, goal_local = false;
-    uint64_t digest = 0;
-    LaneQuery q{p.qvar, p.qinit, p.qinit};
-
-    for (;;) {
-        if (!active && !done) {
-            unsigned int r = 0;
-            if (sl == 0) r = atomicAdd(&chunk_next, 1u);
-            r = __shfl(r, seg_base);
-            if (r >= r_end) {
-                done = true;
-            } else {
-                ridx = r;
-                run = p.run_begin + round * p.n_runs + r;
-                active = true;
-                t = 0.0; step = 0; trans = 0; chan = 0; sender = -1;
-                goal = false; goal_local = false;
-                q.qmax = p.qinit; q.qmin = p.qinit;
-                digest = TRACE ? smc_mix(0, run) : 0;
-                for (int v = sl; v < m.n_vars; v += S) vars[v] = m.var_init[v];
-#pragma unroll
-                for (int j = 0; j < C; ++j) {
-                    const int c = sl * C + j;
-                    loc[j] = c < n ? m.comp_init_loc[c] : -1;
-                }
-                wave_sync();
-            }
-        }
-        if (__ballot(!done) == 0) break;
-        if (done) continue;
-
-        // ---- loop test (simulation.cu:835-849)
-        bool hit = false, end = false;
-        if (!(t < p.bound)) end = true;
-        else if (p.reach && goal) { end = true; hit = true; }
-        else if (step >= p.max_steps) end = true;
-
-        if (!end) {
-            double d = 0.0;
-            int w = -1;
-            // phase 0: pending broadcast receivers (SyncInSerial simulation.cu:357-407)
-            // phase 1: race (compute_possible_delay :409-541, find_minimum_delay
-            //          :544-671), clock advance, the winner's transition
-#pragma clang loop unroll(disable)
-            for (int phase = 0; phase < 2; ++phase) {
-                if (phase == 0 && chan <= 0) continue;
-                if (phase == 1) {
-                    double leaf[C], Mb[C];
-                    int zero_j = C;
-                    uint32_t bmask = 0;
-#pragma unroll
-                    for (int j = 0; j < C; ++j) {
-                        leaf[j] = 0.0;
-                        Mb[j] = 0.0;
-                        const int c = sl * C + j;
-                        if (c < n) {
-                            const DevLoc L = m.locs[loc[j]];
-                            if (L.klass == SMC_K_ZERO) {
-                                if (zero_j == C) zero_j = j;
-                            } else {
-                                bool bounded = false;
-                                double M = DBL_MAX;
-                                for (int i = 0; i < L.n_inv; ++i) {
-                                    const DevInv iv = m.invs[L.inv_first + i];
-                                    const double bound = iv.bound_const ? iv.bound_value : eval_ref(iv.bound, vars, m);
-                                    double ttb = bound - vars[iv.var];
-                                    if (iv.strict) ttb -= 1e-6;
-                                    if (ttb >= 0.0) { M = ttb < M ? ttb : M; bounded = true; }
-                                }
-                                if (bounded) {
-                                    if (M == 0.0) { if (zero_j == C) zero_j = j; }
-                                    else { Mb[j] = M; bmask |= 1u << j; }
-                                } else {
-                                    const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
-                                    leaf[j] = r > 0.0 ? r : 0.0;
-                                }
-                            }
-                        }
-                    }
-                    const uint64_t zb = __ballot(zero_j < C) & seg_mask;
-                    if (zb) {
-                        const int Lz = lowest_lane(zb);
-                        w = (Lz - seg_base) * C + __shfl(zero_j, Lz);
-                        d = 0.0;
-                    } else {
-                        // uniform delays under clock bounds, lowest slot first on ties
-                        double umin = INF;
-                        int unif_j = -1;
-                        for (uint32_t b = bmask; b; b &= b - 1) {
-                            const int j = __ffs(b) - 1;
-                            const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_UNIFORM, (uint32_t)(sl * C + j));
-                            const double du = slot_get<C>(Mb, j) * smc_u01(rw.v[0], rw.v[1]);
-                            if (du < umin) { umin = du; unif_j = j; }
-                        }
-                        // pairwise tree over the lane's C leaves (heap order, root tr[1])
-                        double tr[2 * C];
-#pragma unroll
-                        for (int j = 0; j < C; ++j) tr[C + j] = leaf[j];
-#pragma unroll
-                        for (int i = C - 1; i >= 1; --i) tr[i] = tr[2 * i] + tr[2 * i + 1];
-                        double s = (C == 1) ? leaf[0] : tr[1];
-                        double own[6], sib[6];
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) {
-                            own[k] = s;
-                            sib[k] = 0.0;
-                            if ((1 << k) < S) {
-                                sib[k] = seg_xchg(s, k);
-                                s = s + sib[k];
-                            }
-                        }
-                        const double R = s;
-                        double dexp = INF;
-                        int wexp = 0x7FFFFFFF;
-                        if (R > 0.0) {
-                            const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_RACE, 0);
-                            dexp = -smc_log2(smc_u01(rw.v[0], rw.v[1])) / R;
-                            double v = smc_u01(rw.v[2], rw.v[3]) * R;
-                            int b = 0;
-#pragma unroll
-                            for (int k = 5; k >= 0; --k) {
-                                if ((1 << k) < S) {
-                                    const bool in_left = ((sl >> k) & 1) == 0;
-                                    const double left = in_left ? own[k] : sib[k];
-                                    const double right = in_left ? sib[k] : own[k];
-                                    if (!(v < left || right == 0.0)) { v = v - left; b += 1 << k; }
-                                }
-                            }
-                            int node = 1;
-#pragma unroll
-                            for (int lv = 0; (1 << lv) < C; ++lv) {
-                                double left = 0.0, right = 0.0;
-#pragma unroll
-                                for (int cand = (1 << lv); cand < (2 << lv); ++cand)
-                                    if (node == cand) { left = tr[2 * cand]; right = tr[2 * cand + 1]; }
-                                if (v < left || right == 0.0) node = 2 * node;
-                                else { v = v - left; node = 2 * node + 1; }
-                            }
-                            const int Lw = lowest_lane(__ballot(sl == b) & seg_mask);
-                            wexp = (Lw - seg_base) * C + __shfl(node - C, Lw);
-                        }
-                        double gu = umin;
-#pragma unroll
-                        for (int k = 0; k < 6; ++k)
-                            if ((1 << k) < S) gu = fmin(gu, seg_xchg(gu, k));
-                        int wu = 0x7FFFFFFF;
-                        if (gu < INF) {
-                            const int Lu = lowest_lane(__ballot(unif_j >= 0 && umin == gu) & seg_mask);
-                            wu = (Lu - seg_base) * C + __shfl(unif_j, Lu);
-                        }
-                        if (gu < dexp || (gu == dexp && wu < wexp)) { d = gu; w = wu; }
-                        else { d = dexp; w = wexp; }
-                    }
-                    if (!(d < INF)) { t = INF; end = true; break; }
-                    // clocks advance (find_minimum_delay :632-651)
-                    for (int i = sl; i < m.n_clocks; i += S) vars[m.clocks[i]] += d;
-                    wave_sync();
-                    // the winner's transition (check_enabled_edges :268-305,
-                    // take_transition :27-181), once, on the lane holding it
-                    const int wl = seg_base + w / C;
-                    int e_fired = -1, new_chan = 0;
-                    if (lane == wl) {
-                        const int wj = w % C;
-                        const int lj = slot_get<C>(loc, wj);
-                        const uint64_t mkw = enabled_mask(lj, 0, vars, m);
-                        if (mkw) {
-                            const int first = (int)m.locs[lj].edge_first;
-                            const int i = pick_edge(mkw, first, p.seed, run, step, SMC_P_WINNER, (uint32_t)w, vars, m);
-                            e_fired = first + i;
-                            slot_set<C>(loc, wj, fire_edge(e_fired, vars, m, q, goal_local, new_chan));
-                        }
-                    }
-                    wave_sync();
-                    e_fired = __shfl(e_fired, wl);
-                    if (e_fired >= 0) {
-                        ++trans;
-                        if (TRACE)
-                            digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)w << 20) | (uint64_t)e_fired);
-                        chan = __shfl(new_chan, wl);
-                        sender = chan > 0 ? w : -1;
-                    }
-                    goal = (__ballot(goal_local) & seg_mask) != 0;
-                    break;
-                }
-
-                // phase 0: enabled receive edges of every component but the sender
-                uint64_t mk[C];
-                bool any = false;
-#pragma unroll
-                for (int j = 0; j < C; ++j) {
-                    const int c = sl * C + j;
-                    mk[j] = (c < n && c != sender) ? enabled_mask(loc[j], chan, vars, m) : 0ull;
-                    any |= mk[j] != 0;
-                }
-                // fire them one at a time in component order
-                int dummy_chan = 0;
-                for (uint64_t lanes_with = __ballot(any) & seg_mask; lanes_with; lanes_with &= lanes_with - 1) {
-                    const int L = lowest_lane(lanes_with);
-#pragma clang loop unroll(disable)
-                    for (int j = 0; j < C; ++j) {
-                        if (!((__ballot(slot_get<C>(mk, j) != 0) >> L) & 1ull)) continue;
-                        const int c = (L - seg_base) * C + j;
-                        int e_fired = -1;
-                        if (lane == L) {
-                            const int lj = slot_get<C>(loc, j);
-                            const int first = (int)m.locs[lj].edge_first;
-                            const int i = pick_edge(slot_get<C>(mk, j), first, p.seed, run, step, SMC_P_RECEIVER,
-                                                    (uint32_t)c, vars, m);
-                            e_fired = first + i;
-                            slot_set<C>(loc, j, fire_edge(e_fired, vars, m, q, goal_local, dummy_chan));
-                        }
-                        wave_sync();
-                        ++trans;
-                        if (TRACE) {
-                            const int e = __shfl(e_fired, L);
-                            digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)c << 20) | (uint64_t)e);
-                        }
-                    }
-                }
-                chan = 0;
-                sender = -1;
-                goal = (__ballot(goal_local) & seg_mask) != 0;
-            }
-            if (!end) { t = t + d; ++step; }
-        }
-
-        if (end) {
-            // per-run result (simulation.cu:887-893, statistics.cuh:126-163)
-            double qmax = q.qmax, qmin = q.qmin;
-#pragma unroll
-            for (int k = 0; k < 6; ++k)
-                if ((1 << k) < S) {
-                    qmax = fmax(qmax, seg_xchg(qmax, k));
-                    qmin = fmin(qmin, seg_xchg(qmin, k));
-                }
-            const double value = p.is_max ? qmax : qmin;
-            if (TRACE) {
-                digest = smc_mix(digest, smc_digest_bits(t));
-                for (int v = 0; v < m.n_vars; ++v) digest = smc_mix(digest, smc_digest_bits(vars[v]));
-                for (int c = 0; c < n; ++c) {
-                    const int lv = __shfl(slot_get<C>(loc, c % C), seg_base + c / C);
-                    digest = smc_mix(digest, (uint64_t)(uint32_t)lv);
-                }
-            }
-            if (sl == 0) {
-                uint8_t flag;
-                if (p.reach) flag = hit ? 1 : 0;
-                else flag = p.is_max ? (value > p.threshold) : (value < p.threshold);
-                p.flag[ridx] = flag;
-                p.value[ridx] = value;
-                p.steps[ridx] = step;
-                p.trans[ridx] = trans;
-                if (p.time) p.time[ridx] = t;
-                if (TRACE && p.digest) p.digest[ridx] = digest;
-            }
-            wave_sync();
-            active = false;
-        }
-    }
-  }
-}
 
 // ------------------------------------------------------------ reduction
 
@@ -439,34 +97,30 @@ __global__ void prep_kernel(unsigned long long *counter, uint64_t *round, int ad
 
 namespace smc {
 
-template <int C>
-static hipError_t launch_c(const LaunchParams &lp, bool trace, int waves, size_t lds, hipStream_t s) {
-    if (trace) hipLaunchKernelGGL((sim_kernel<C, true>), dim3(waves), dim3(64), lds, s, lp);
-    else hipLaunchKernelGGL((sim_kernel<C, false>), dim3(waves), dim3(64), lds, s, lp);
-    return hipGetLastError();
-}
-
-hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int waves, size_t lds_bytes, hipStream_t stream) {
+static SimVariant variant(int slots, int occ) {
+    if (occ != 0 && occ != 2) return SimVariant{};
+    const bool o2 = occ == 2;
     switch (slots) {
-    case 1: return launch_c<1>(lp, trace, waves, lds_bytes, stream);
-    case 2: return launch_c<2>(lp, trace, waves, lds_bytes, stream);
-    case 4: return launch_c<4>(lp, trace, waves, lds_bytes, stream);
-    case 8: return launch_c<8>(lp, trace, waves, lds_bytes, stream);
-    case 16: return launch_c<16>(lp, trace, waves, lds_bytes, stream);
-    default: return hipErrorInvalidValue;
+    case 1: return o2 ? sim_variant_c1_o2() : sim_variant_c1_o0();
+    case 2: return o2 ? sim_variant_c2_o2() : sim_variant_c2_o0();
+    case 4: return o2 ? sim_variant_c4_o2() : sim_variant_c4_o0();
+    case 8: return o2 ? sim_variant_c8_o2() : sim_variant_c8_o0();
+    case 16: return o2 ? sim_variant_c16_o2() : sim_variant_c16_o0();
+    default: return SimVariant{};
     }
 }
 
-hipError_t sim_occupancy(int slots, bool trace, size_t lds_bytes, int *blocks_per_cu) {
-    const void *f = nullptr;
-#define SMC_PICK(CC)                                                          \
-    case CC: f = trace ? (const void *)sim_kernel<CC, true> : (const void *)sim_kernel<CC, false>; break;
-    switch (slots) {
-        SMC_PICK(1) SMC_PICK(2) SMC_PICK(4) SMC_PICK(8) SMC_PICK(16)
-    default: return hipErrorInvalidValue;
-    }
-#undef SMC_PICK
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 64, lds_bytes);
+hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int occ, int waves, size_t lds_bytes,
+                      hipStream_t stream) {
+    const SimVariant v = variant(slots, occ);
+    if (!v.launch) return hipErrorInvalidValue;
+    return v.launch(lp, trace, waves, lds_bytes, stream);
+}
+
+hipError_t sim_occupancy(int slots, bool trace, int occ, size_t lds_bytes, int *blocks_per_cu) {
+    const SimVariant v = variant(slots, occ);
+    if (!v.occupancy) return hipErrorInvalidValue;
+    return v.occupancy(trace, lds_bytes, blocks_per_cu);
 }
 
 hipError_t launch_reduce(const InstDesc *inst, int n_inst, uint32_t blocks_per_inst, ReducePartial *partials,

@@ -75,7 +75,8 @@ typedef struct smc_launch {
     int32_t lanes_per_run;     /* S: power of two <= 64                    */
     int32_t waves;             /* persistent waves in the grid              */
     int32_t waves_per_cu;      /* occupancy target used when waves == 0     */
-    int32_t reserved;
+    int32_t occupancy;         /* 0: registers unrestricted; 2: at most 256
+                                  registers, two waves per SIMD            */
 } smc_launch;
 
 /* Per-run outputs, all optional (NULL = not wanted), each n_runs long. */
