@@ -40,7 +40,7 @@ sys.path.insert(0, str(ROOT))
 LANE_PEAK = 256 * 64 * 2.4e9        # VALU lane-operations per second
 WAVE_PEAK = 256 * 2.4e9             # wave64 VALU instructions per second
 # algorithmic lane-ops per race step (oracle cost model; tests pin them)
-COST_PER_STEP = {"fischer32": 334.7, "sir350": 1424.7, "race8": 881.2}
+COST_PER_STEP = {"fischer32": 334.7, "sir350": 2626.3, "race8": 881.2}
 SEED = 20261202
 
 

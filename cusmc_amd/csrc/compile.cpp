@@ -18,6 +18,7 @@
 #include "compile.hpp"
 
 #include <cstring>
+#include <functional>
 #include <map>
 #include <stdexcept>
 #include <algorithm>
@@ -290,6 +291,45 @@ ModelImage compile_model(const smc_network &n) {
         if (E.upd_count > 65535) throw std::runtime_error("limit: more than 65535 updates on an edge");
         d.upd_n = (uint16_t)E.upd_count;
         img.edges.push_back(d);
+    }
+    // time_free: the location's rate and its spontaneous edges' guards read
+    // no clock, so between transitions neither its enabled edges nor its
+    // rate change
+    std::vector<int8_t> clock_memo(n.n_exprs, -1);
+    std::function<bool(int)> reads_clock = [&](int x) -> bool {
+        if (x < 0) return false;
+        if (clock_memo[x] >= 0) return clock_memo[x] != 0;
+        const smc_expr_node &nd = n.exprs[x];
+        bool r = nd.op == SMC_E_VAR && n.var_kind[nd.var] == SMC_VAR_CLOCK;
+        if (!r && nd.op != SMC_E_LIT && nd.op != SMC_E_VAR)
+            r = reads_clock(nd.a) || reads_clock(nd.b) || reads_clock(nd.c);
+        clock_memo[x] = r ? 1 : 0;
+        return r;
+    };
+    for (int l = 0; l < n.n_locations; ++l) {
+        const smc_location &L = n.locations[l];
+        bool tf = !reads_clock(L.rate);
+        for (int k = 0; k < L.edge_count && tf; ++k) {
+            const smc_edge &E = n.edges[L.edge_first + k];
+            if (E.channel < 0) continue;
+            for (int g = 0; g < E.guard_count; ++g) {
+                const smc_constraint &G = n.constraints[E.guard_first + g];
+                if (!G.uses_variable) continue;
+                if (n.var_kind[G.var] == SMC_VAR_CLOCK || reads_clock(G.rhs)) tf = false;
+            }
+        }
+        bool dead = img.locs[l].klass == SMC_K_EXP;
+        for (int k = 0; k < L.edge_count && dead; ++k) {
+            const smc_edge &E = n.edges[L.edge_first + k];
+            if (E.channel < 0) continue;
+            bool never = false;
+            for (int g = 0; g < E.guard_count; ++g) {
+                const smc_constraint &G = n.constraints[E.guard_first + g];
+                if (G.uses_variable && G.op == SMC_C_NE) never = true;
+            }
+            if (!never) dead = false;
+        }
+        img.locs[l].flags = (uint8_t)((tf ? SMC_LF_TIME_FREE : 0) | (dead ? SMC_LF_DEAD : 0));
     }
     for (int l = 0; l < n.n_locations; ++l) {
         const smc_location &L = n.locations[l];

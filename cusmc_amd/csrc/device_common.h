@@ -54,6 +54,22 @@ __device__ __forceinline__ double seg_xchg(double x, int k) {
 }
 
 
+// Pairwise tree sum over the segment, as in the run kernel (in-lane levels,
+// then the butterfly); every lane gets the total.
+template <int C>
+__device__ __forceinline__ double tree_total(const double (&leaf)[C], int S) {
+    double tr[2 * C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) tr[C + j] = leaf[j];
+#pragma unroll
+    for (int i = C - 1; i >= 1; --i) tr[i] = tr[2 * i] + tr[2 * i + 1];
+    double s = (C == 1) ? leaf[0] : tr[1];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        if ((1 << k) < S) s = s + seg_xchg(s, k);
+    return s;
+}
+
 // ------------------------------------------------------------ expressions
 
 // Register-bytecode interpreter (expressions.cu:10-119).  Inlined: an
@@ -153,6 +169,27 @@ __device__ __forceinline__ int pick_edge(uint64_t mask, int edge_first, uint64_t
     return lowest_lane(mask);
 }
 
+// Rate sum, on the race tree, of the components that cannot fire
+// (SMC_LF_DEAD) in the slots of deadmask.  They leave the race when every
+// racer is exponential; the transition that crosses the time bound still
+// competes with their first null event after the bound, which is drawn from
+// this sum (memoryless races).
+template <int C>
+__device__ __forceinline__ double dead_total(uint32_t deadmask, const int (&loc)[C], const double *vars,
+                                             const DevModel &m, int S) {
+    double dl[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        dl[j] = 0.0;
+        if ((deadmask >> j) & 1u) {
+            const DevLoc L = m.locs[loc[j]];
+            const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
+            dl[j] = r > 0.0 ? r : 0.0;
+        }
+    }
+    return tree_total<C>(dl, S);
+}
+
 struct LaneQuery {
     int qvar;
     double qmax, qmin;
@@ -175,6 +212,30 @@ __device__ __forceinline__ int fire_edge(int e, double *vars, const DevModel &m,
     if (E.dst_goal) goal = true;
     if (E.channel > 0) chan = E.channel;
     return E.dst;
+}
+
+// Whether no component of the segment's run can fire again before the time
+// bound: each one is in a location whose rate and spontaneous guards read no
+// clock, and it has no enabled spontaneous edge or races at rate 0 (an
+// exponential location).  Zero-delay and clock-bounded locations with an
+// enabled edge can fire.  Nothing but time changes such a state, so the run's
+// outcome is decided; the kernels check it after every change of state.
+template <int C>
+__device__ __forceinline__ bool absorbed(const int (&loc)[C], const double *vars, const DevModel &m, int sl, int n,
+                                         uint64_t seg_mask) {
+    bool can = false;
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        const int c = sl * C + j;
+        if (can || c >= n) continue;
+        const DevLoc L = m.locs[loc[j]];
+        if (!(L.flags & SMC_LF_TIME_FREE) || L.klass == SMC_K_ZERO) { can = true; continue; }
+        if (enabled_mask(loc[j], 0, vars, m) == 0) continue;
+        if (L.klass != SMC_K_EXP) { can = true; continue; }
+        const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
+        if (r > 0.0) can = true;
+    }
+    return (__ballot(can) & seg_mask) == 0;
 }
 
 template <int C, class T>

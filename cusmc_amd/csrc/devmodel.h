@@ -44,6 +44,14 @@ enum smc_delay_class {
     SMC_K_BOUNDED = 2   // uniform up to the nearest clock upper bound, else exponential
 };
 
+// Location flags.
+enum {
+    SMC_LF_TIME_FREE = 1,   // rate and spontaneous guards read no clock: enabling and rate
+                            // change only with transitions
+    SMC_LF_DEAD = 2         // exponential, and no spontaneous edge can ever fire: its races
+                            // are null steps
+};
+
 struct DevInsn {
     uint8_t op;
     uint8_t dst;
@@ -60,7 +68,7 @@ struct DevLoc {             // 32 bytes, one pair of 16-byte loads
     double rate_value;      // the rate when it is a constant (rate_const)
     uint16_t edge_n;
     uint8_t rate_const;
-    uint8_t pad0;
+    uint8_t flags;          // SMC_LF_*
     uint32_t recv_fast;     // bit i: edge i is a commuting receive edge (see DevUpd)
 };
 

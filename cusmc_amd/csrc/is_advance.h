@@ -46,22 +46,6 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// Pairwise tree sum over the segment, as in the run kernel (in-lane levels,
-// then the butterfly); every lane gets the total.
-template <int C>
-__device__ __forceinline__ double tree_total(const double (&leaf)[C], int S) {
-    double tr[2 * C];
-#pragma unroll
-    for (int j = 0; j < C; ++j) tr[C + j] = leaf[j];
-#pragma unroll
-    for (int i = C - 1; i >= 1; --i) tr[i] = tr[2 * i] + tr[2 * i + 1];
-    double s = (C == 1) ? leaf[0] : tr[1];
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-        if ((1 << k) < S) s = s + seg_xchg(s, k);
-    return s;
-}
-
 // Features of a run: one-hot of each component's location, scaled
 // variables, time / bound.  Each entry is written by exactly one lane.
 template <int C>
@@ -154,6 +138,7 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
     }
     bool first_race = true;
     bool hit = false, end = false, fired_any = false;
+    bool changed = true;  // state changed since the last absorption check
 
     for (;;) {
         if (!mid) {
@@ -217,9 +202,13 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
             if (phase == 0 && chan <= 0) continue;
             bool exp_won = false;
             if (phase == 1) {
+                if (changed) {
+                    changed = false;
+                    if (absorbed<C>(loc, vars, m, sl, n, seg_mask)) { end = true; break; }
+                }
                 double leaf[C], leaft[C], Mb[C];
                 int zero_j = C;
-                uint32_t bmask = 0;
+                uint32_t bmask = 0, deadmask = 0;
 #pragma unroll
                 for (int j = 0; j < C; ++j) {
                     leaf[j] = 0.0; leaft[j] = 0.0; Mb[j] = 0.0;
@@ -241,6 +230,8 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                             if (bounded) {
                                 if (M == 0.0) { if (zero_j == C) zero_j = j; }
                                 else { Mb[j] = M; bmask |= 1u << j; }
+                            } else if (L.flags & SMC_LF_DEAD) {
+                                deadmask |= 1u << j;
                             } else {
                                 const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
                                 leaf[j] = r > 0.0 ? r : 0.0;
@@ -249,6 +240,22 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                                     leaft[j] = leaf[j] * smc_exp2((double)theta[loc[j]]);
                             }
                         }
+                    }
+                }
+                // components that cannot fire race only beside a uniform
+                // racer (as in the run kernel)
+                const bool uniform = (__ballot(bmask != 0) & seg_mask) != 0;
+                if (uniform) {
+                    for (uint32_t b = deadmask; b; b &= b - 1) {
+                        const int j = __ffs(b) - 1;
+                        const int lj = slot_get<C>(loc, j);
+                        const DevLoc L = m.locs[lj];
+                        const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
+                        const double lf = r > 0.0 ? r : 0.0;
+                        double lt = lf;
+                        if (theta && lf > 0.0 && P.tiltable[lj]) lt = lf * smc_exp2((double)theta[lj]);
+                        slot_set<C>(leaf, j, lf);
+                        slot_set<C>(leaft, j, lt);
                     }
                 }
                 if (first_race) {
@@ -326,6 +333,16 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                     }
                     if (gu < dexp || (gu == dexp && wu < wexp)) { d = gu; w = wu; }
                     else { d = dexp; w = wexp; exp_won = true; }
+                    // the step that crosses the bound: the left-out null
+                    // racers' first event after it (base rates, no tilt)
+                    if (!uniform && d < INF && !(t + d < P.bound) && (__ballot(deadmask != 0) & seg_mask)) {
+                        const double Rd = dead_total<C>(deadmask, loc, vars, m, S);
+                        if (Rd > 0.0) {
+                            const smc_u32x4 rw = smc_draw(P.seed, run, step, SMC_P_DEAD, 0);
+                            const double td = P.bound + (-smc_log2(smc_u01(rw.v[0], rw.v[1])) / Rd);
+                            if (td < t + d) { d = td - t; w = -1; exp_won = false; }
+                        }
+                    }
                     if (theta && d < INF) {
                         // log2 LR of this step: -theta_w + (R' - R) d
                         double th_w = 0.0;
@@ -352,6 +369,7 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                 if (!(d < INF)) { t = INF; end = true; break; }
                 for (int i = sl; i < m.n_clocks; i += S) vars[m.clocks[i]] += d;
                 wave_sync();
+                if (w < 0) break;  // a null event of the left-out racers
                 // the winner's transition, once, on the lane holding it
                 const int wl = seg_base + w / C;
                 int e_fired = -1, new_chan = 0;
@@ -370,6 +388,7 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                 e_fired = __shfl(e_fired, wl);
                 if (e_fired >= 0) {
                     ++trans;
+                    changed = true;
                     if (P.trace)
                         digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)w << 20) | (uint64_t)e_fired);
                     chan = __shfl(new_chan, wl);
@@ -380,6 +399,7 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                 break;
             }
             // phase 0: pending broadcast receivers
+            changed = true;
             uint64_t mk[C];
             bool any = false;
 #pragma unroll

@@ -99,6 +99,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
     uint32_t step = 0, trans = 0;
     int chan = 0, sender = -1;
     bool goal = false, goal_local = false;
+    bool changed = true;  // state changed since the last absorption check
     uint64_t digest = 0;
     LaneQuery q{p.qvar, p.qinit, p.qinit};
 
@@ -114,7 +115,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                 run = p.run_begin + round * p.n_runs + r;
                 active = true;
                 t = 0.0; step = 0; trans = 0; chan = 0; sender = -1;
-                goal = false; goal_local = false;
+                goal = false; goal_local = false; changed = true;
                 q.qmax = p.qinit; q.qmin = p.qinit;
                 digest = TRACE ? smc_mix(0, run) : 0;
                 for (int v = sl; v < m.n_vars; v += S) vars[v] = m.var_init[v];
@@ -145,9 +146,13 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
             for (int phase = 0; phase < 2; ++phase) {
                 if (phase == 0 && chan <= 0) continue;
                 if (phase == 1) {
+                    if (changed) {
+                        changed = false;
+                        if (absorbed<C>(loc, vars, m, sl, n, seg_mask)) { end = true; break; }
+                    }
                     double leaf[C], Mb[C];
                     int zero_j = C;
-                    uint32_t bmask = 0;
+                    uint32_t bmask = 0, deadmask = 0;
 #pragma unroll
                     for (int j = 0; j < C; ++j) {
                         leaf[j] = 0.0;
@@ -170,11 +175,27 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                                 if (bounded) {
                                     if (M == 0.0) { if (zero_j == C) zero_j = j; }
                                     else { Mb[j] = M; bmask |= 1u << j; }
+                                } else if (L.flags & SMC_LF_DEAD) {
+                                    deadmask |= 1u << j;
                                 } else {
                                     const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
                                     leaf[j] = r > 0.0 ? r : 0.0;
                                 }
                             }
+                        }
+                    }
+                    // A component whose location cannot fire only makes null
+                    // steps.  With every racer exponential, leaving it out of
+                    // the race keeps the law of everything else (memoryless
+                    // races); a uniform racer is redrawn at every step, so then
+                    // it races as in the reference.
+                    const bool uniform = (__ballot(bmask != 0) & seg_mask) != 0;
+                    if (uniform) {
+                        for (uint32_t b = deadmask; b; b &= b - 1) {
+                            const int j = __ffs(b) - 1;
+                            const DevLoc L = m.locs[slot_get<C>(loc, j)];
+                            const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
+                            slot_set<C>(leaf, j, r > 0.0 ? r : 0.0);
                         }
                     }
                     const uint64_t zb = __ballot(zero_j < C) & seg_mask;
@@ -260,11 +281,22 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                         }
                         if (gu < dexp || (gu == dexp && wu < wexp)) { d = gu; w = wu; }
                         else { d = dexp; w = wexp; }
+                        // the step that crosses the bound: the left-out null
+                        // racers' first event after the bound may come first
+                        if (!uniform && d < INF && !(t + d < p.bound) && (__ballot(deadmask != 0) & seg_mask)) {
+                            const double Rd = dead_total<C>(deadmask, loc, vars, m, S);
+                            if (Rd > 0.0) {
+                                const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_DEAD, 0);
+                                const double td = p.bound + (-smc_log2(smc_u01(rw.v[0], rw.v[1])) / Rd);
+                                if (td < t + d) { d = td - t; w = -1; }
+                            }
+                        }
                     }
                     if (!(d < INF)) { t = INF; end = true; break; }
                     // clocks advance (find_minimum_delay :632-651)
                     for (int i = sl; i < m.n_clocks; i += S) vars[m.clocks[i]] += d;
                     wave_sync();
+                    if (w < 0) break;  // a null event of the left-out racers
                     // the winner's transition (check_enabled_edges :268-305,
                     // take_transition :27-181), once, on the lane holding it
                     const int wl = seg_base + w / C;
@@ -284,6 +316,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                     e_fired = __shfl(e_fired, wl);
                     if (e_fired >= 0) {
                         ++trans;
+                        changed = true;
                         if (TRACE)
                             digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)w << 20) | (uint64_t)e_fired);
                         chan = __shfl(new_chan, wl);
@@ -294,6 +327,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                 }
 
                 // phase 0: enabled receive edges of every component but the sender
+                changed = true;
                 uint64_t mk[C];
                 bool any = false;
 #pragma unroll

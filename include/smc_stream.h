@@ -78,7 +78,9 @@ enum smc_purpose {
     SMC_P_UNIFORM = 1,  /* per component: delay under a clock upper bound     */
     SMC_P_WINNER = 2,   /* per component: weighted edge choice of the winner  */
     SMC_P_RECEIVER = 3, /* per component: weighted choice of a broadcast receiver */
-    SMC_P_TILT = 4      /* importance sampling: spare stream                  */
+    SMC_P_TILT = 4,     /* importance sampling: spare stream                  */
+    SMC_P_DEAD = 5      /* per run and step: first null event after the time
+                           bound of the components that cannot fire        */
 };
 
 SMC_HD smc_u32x4 smc_draw(uint64_t seed, uint64_t run, uint32_t step,

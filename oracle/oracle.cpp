@@ -23,6 +23,10 @@
 //     t += delay                                        simulation.cu:876-882
 //   a run is cut off after max_steps race rounds (no counterpart: cuSMC
 //     spins forever on a cycle of zero delays)
+//   CANONICAL only: a run ends when no component can fire again before the
+//     bound (absorption: every component has no enabled spontaneous edge or
+//     races at rate 0, in a location whose rate and guards read no clock);
+//     cuSMC races null steps to the bound there, with the same outcome
 //   guards: only conjuncts on a variable; `==` within 1e-5; `!=` disables
 //     the edge                                          simulation.cu:184-266
 //
@@ -188,6 +192,43 @@ public:
         recv_.resize(n.n_components);
         for (int v = 0; v < n.n_vars; ++v)
             if (n.var_kind[v] == SMC_VAR_CLOCK) clocks_.push_back(v);
+        // per location: rate and spontaneous guards free of clocks; clock
+        // upper bounds present (the conditions of absorption below)
+        time_free_.assign(n.n_locations, 1);
+        clock_bounded_.assign(n.n_locations, 0);
+        dead_.assign(n.n_locations, 0);
+        for (int l = 0; l < n.n_locations; ++l) {
+            const smc_location &L = n.locations[l];
+            if (L.rate >= 0 && reads_clock(L.rate)) time_free_[l] = 0;
+            for (int k = 0; k < L.edge_count; ++k) {
+                const smc_edge &E = n.edges[L.edge_first + k];
+                if (E.channel < 0) continue;
+                for (int g = 0; g < E.guard_count; ++g) {
+                    const smc_constraint &G = n.constraints[E.guard_first + g];
+                    if (G.uses_variable && (n.var_kind[G.var] == SMC_VAR_CLOCK || reads_clock(G.rhs)))
+                        time_free_[l] = 0;
+                }
+            }
+            for (int k = 0; k < L.inv_count; ++k) {
+                const smc_constraint &I = n.constraints[L.inv_first + k];
+                if (I.uses_variable && n.var_kind[I.var] == SMC_VAR_CLOCK && (I.op == SMC_C_LE || I.op == SMC_C_LT))
+                    clock_bounded_[l] = 1;
+            }
+            // dead: exponential, and every spontaneous edge has a `!=` guard
+            // (never enabled) or there is none
+            bool dead = L.type <= SMC_LOC_GOAL && !clock_bounded_[l];
+            for (int k = 0; k < L.edge_count && dead; ++k) {
+                const smc_edge &E = n.edges[L.edge_first + k];
+                if (E.channel < 0) continue;
+                bool never = false;
+                for (int g = 0; g < E.guard_count; ++g) {
+                    const smc_constraint &G = n.constraints[E.guard_first + g];
+                    if (G.uses_variable && G.op == SMC_C_NE) never = true;
+                }
+                if (!never) dead = false;
+            }
+            dead_[l] = dead ? 1 : 0;
+        }
     }
 
     // events: optional log of (step, component, edge) per fired edge
@@ -213,6 +254,7 @@ public:
         uint32_t step = 0, trans = 0;
         int chan = 0, sender = -1;
         bool goal = false, hit = false;
+        bool changed = true;  // state changed since the last absorption check
         uint64_t digest = smc_mix(0, run);
         auto log = [&](int c, int e) {
             ++trans;
@@ -261,6 +303,7 @@ public:
                 }
                 chan = 0;
                 sender = -1;
+                changed = true;
             }
 
             if (record && new_macro) {
@@ -282,12 +325,18 @@ public:
                 base_rates_ = &r0;
             }
             new_macro = false;
+            // absorption (canonical sampling; cuSMC keeps racing null
+            // steps to the bound, with the same outcome)
+            if (mode_ == ORACLE_CANONICAL && changed) {
+                changed = false;
+                if (absorbed()) break;
+            }
 
             double d;
             int w;
             bool exp_won = false;
             if (mode_ == ORACLE_LITERAL) race_literal(run, step, &d, &w);
-            else race_canonical(run, step, &d, &w, &exp_won);
+            else race_canonical(run, step, t, &d, &w, &exp_won);
             base_rates_ = nullptr;
             if (record && exp_won) ++wins[w];
             if (!(d < HUGE_VAL)) { t = HUGE_VAL; break; }
@@ -295,7 +344,7 @@ public:
             for (int v : clocks_) vars_[v] += d;
             cost(COST_OP * (double)clocks_.size() + 2 * COST_OP);  // clocks, time
             std::vector<int> en;
-            enabled(loc_[w], 0, &en);
+            if (w >= 0) enabled(loc_[w], 0, &en);
             if (!en.empty()) {
                 smc_u32x4 r = smc_draw(seed_, run, step, SMC_P_WINNER, (uint32_t)w);
                 int e = pick(en, smc_u01(r.v[0], r.v[1]));
@@ -304,6 +353,7 @@ public:
                 log(w, e);
                 if (nc > 0) { chan = nc; sender = w; }
                 new_macro = true;
+                changed = true;
             }
             t = t + d;
             ++step;
@@ -333,13 +383,41 @@ private:
     std::vector<double> leaf_, vars_;
     std::vector<int> loc_, clocks_;
     std::vector<std::vector<int>> recv_;
-    RaceTree tree_, base_tree_;
+    RaceTree tree_, base_tree_, dead_tree_;
+    std::vector<double> dleaf_;
     std::vector<double> tilted_;
     const IsOpts *is_ = nullptr;
     std::vector<double> *base_rates_ = nullptr;  // receives the base rates of this race
     double log2lr_ = 0.0;
     int qvar_ = -1;
     double qmax_ = 0, qmin_ = 0;
+
+    std::vector<uint8_t> time_free_, clock_bounded_, dead_;
+
+    bool reads_clock(int x) const {
+        if (x < 0) return false;
+        const smc_expr_node &nd = n_.exprs[x];
+        if (nd.op == SMC_E_VAR) return n_.var_kind[nd.var] == SMC_VAR_CLOCK;
+        if (nd.op == SMC_E_LIT) return false;
+        return reads_clock(nd.a) || reads_clock(nd.b) || reads_clock(nd.c);
+    }
+
+    // No component can fire before the time bound: each sits in a location
+    // whose rate and spontaneous guards read no clock, with no enabled
+    // spontaneous edge or an exponential rate of 0.  Only time changes such
+    // a state, so the outcome is decided.
+    bool absorbed() const {
+        for (int c = 0; c < n_.n_components; ++c) {
+            const int l = loc_[c];
+            if (n_.locations[l].type > SMC_LOC_GOAL || !time_free_[l]) return false;
+            std::vector<int> en;
+            enabled(l, 0, &en);
+            if (en.empty()) continue;
+            if (clock_bounded_[l]) return false;
+            if (rate_of(c) > 0.0) return false;
+        }
+        return true;
+    }
 
     bool guards_hold(const smc_edge &E) const {
         for (int g = 0; g < E.guard_count; ++g) {
@@ -437,11 +515,13 @@ private:
         return L.rate >= 0 ? eval(n_, L.rate, vars_.data()) : 1.0;
     }
 
-    void race_canonical(uint64_t run, uint32_t step, double *d_out, int *w_out, bool *exp_won) {
+    // t: the time before the step (for the crossing of the bound)
+    void race_canonical(uint64_t run, uint32_t step, double t, double *d_out, int *w_out, bool *exp_won) {
         const int N = n_.n_components;
         int zero = -1, wu = -1;
         double du = HUGE_VAL;
         cost(2 * COST_OP * N);  // delay class and leaf of every component
+        bool uniform = false;
         for (int c = 0; c < N; ++c) {
             leaf_[c] = 0.0;
             const int type = n_.locations[loc_[c]].type;
@@ -453,11 +533,22 @@ private:
                 smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_UNIFORM, (uint32_t)c);
                 const double d = M * smc_u01(w.v[0], w.v[1]);
                 if (d < du) { du = d; wu = c; }
-            } else {
+                uniform = true;
+            } else if (!dead_[loc_[c]]) {
                 const double r = rate_of(c);
                 leaf_[c] = r > 0.0 ? r : 0.0;
             }
         }
+        // A component that cannot fire only makes null steps: with every
+        // racer exponential it leaves the race without changing the law of
+        // the rest (memoryless races); beside a uniform racer, which is
+        // redrawn at every step, it races as in cuSMC.
+        if (uniform)
+            for (int c = 0; c < N; ++c)
+                if (n_.locations[loc_[c]].type <= SMC_LOC_GOAL && dead_[loc_[c]] && !clock_bounded_[loc_[c]]) {
+                    const double r = rate_of(c);
+                    leaf_[c] = r > 0.0 ? r : 0.0;
+                }
         if (base_rates_) for (int c = 0; c < N; ++c) (*base_rates_)[c] = leaf_[c];
         if (zero >= 0) { *d_out = 0.0; *w_out = zero; return; }
         {
@@ -491,6 +582,28 @@ private:
         bool won = false;
         if (du < dexp || (du == dexp && wu < wexp)) { *d_out = du; *w_out = wu; }
         else { *d_out = dexp; *w_out = wexp; won = true; }
+        // the step that crosses the bound: the left-out null racers' first
+        // event after the bound (base rates) may come first; then the step
+        // is a null step (w = -1)
+        if (!uniform && *d_out < HUGE_VAL && !(t + *d_out < q_.time_bound)) {
+            bool any = false;
+            dleaf_.assign(N, 0.0);
+            for (int c = 0; c < N; ++c)
+                if (n_.locations[loc_[c]].type <= SMC_LOC_GOAL && dead_[loc_[c]]) {
+                    const double r = rate_of(c);
+                    dleaf_[c] = r > 0.0 ? r : 0.0;
+                    any = true;
+                }
+            if (any) {
+                dead_tree_.build(dleaf_, N);
+                const double Rd = dead_tree_.t[1];
+                if (Rd > 0.0) {
+                    smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_DEAD, 0);
+                    const double td = q_.time_bound + (-smc_log2(smc_u01(w.v[0], w.v[1])) / Rd);
+                    if (td < t + *d_out) { *d_out = td - t; *w_out = -1; won = false; }
+                }
+            }
+        }
         if (exp_won) *exp_won = won;
         if (theta && *d_out < HUGE_VAL) {
             // log2 of the likelihood ratio of this race: -theta_w (exponential
