@@ -195,7 +195,8 @@ def bench_is(args) -> dict:
     plain_to_ci_s = plain_s if rel <= 0.1 else need / plain_rate
     # learned IS: create, train (5 rounds x 1e5), estimate to +-10%
     t1 = time.perf_counter()
-    s = cs.ImportanceSampler(sim, q, rounds=5, runs_per_round=100_000, epochs=args.epochs, seed=SEED)
+    s = cs.ImportanceSampler(sim, q, rounds=5, runs_per_round=100_000, epochs=args.epochs,
+                             learning_rate=args.lr, seed=SEED)
     rounds = s.train()
     t_train = time.perf_counter() - t1
     est = s.estimate(seed=SEED + 1, batch_runs=65536, max_runs=1 << 24, rel_target=0.1)
@@ -211,7 +212,7 @@ def bench_is(args) -> dict:
         "dtype": "f64 (simulation), bf16/fp32 (MLP)",
         "data": "synthetic (SIR N=100 network generated from the seed)",
         "config": {"workload": "is sir-100 Pr[<=50](<> I>=90)", "exact": exact, "rounds": 5,
-                   "runs_per_round": 100_000, "epochs": args.epochs},
+                   "runs_per_round": 100_000, "epochs": args.epochs, "learning_rate": args.lr},
         "is_seconds": is_s, "is_train_seconds": t_train, "is_estimate": est.estimate,
         "is_rel_half_width": est.rel_half_width, "is_runs": est.n_runs, "is_hits": est.hits, "is_ess": est.ess,
         "plain_seconds_to_ci": plain_to_ci_s, "plain_measured_seconds": plain_s, "plain_runs": n,
@@ -298,7 +299,8 @@ def main() -> int:
     ap.add_argument("--runs", type=int, default=0, help="runs per step (default: the workload's)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--plain-cap-s", type=float, default=120.0, help="is: cap on the plain run's wall time")
-    ap.add_argument("--epochs", type=int, default=20, help="is: epochs per cross-entropy round")
+    ap.add_argument("--epochs", type=int, default=6, help="is: epochs per cross-entropy round")
+    ap.add_argument("--lr", type=float, default=1e-2, help="is: Adam step")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -71,7 +71,8 @@ def build_lib() -> Path:
         list(ex.map(_run, jobs))
     out = LIB / "libcusmc.so"
     tmp = LIB / "libcusmc.so.tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *[str(o) for o in objs]])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", str(tmp),
+          *[str(o) for o in objs]])
     os.replace(tmp, out)
     return out
 

@@ -78,9 +78,56 @@ uint16_t host_bf16(float f) {
 
 enum class Tilt { NONE, FIXED, MLP };
 
+// Row state of a lock-step batch.  Two sets: compaction copies the rows
+// still running from one into the other.
+struct RowSet {
+    DevBuf<uint16_t> loc;
+    DevBuf<double> vars, t, log2lr, qmax, qmin;
+    DevBuf<uint32_t> step, trans, job;
+    DevBuf<int32_t> chan, sender;
+    DevBuf<uint8_t> status;
+    DevBuf<uint64_t> digest, run;
+    DevBuf<uint16_t> X;  // bf16 features of the rows' current states
+
+    void ensure(size_t r, int n_comp, int n_vars, int fpad, bool feats) {
+        loc.ensure(r * n_comp, "hipMalloc rows");
+        vars.ensure(r * (size_t)std::max(n_vars, 1), "hipMalloc rows");
+        for (DevBuf<double> *b : {&t, &log2lr, &qmax, &qmin}) b->ensure(r, "hipMalloc rows");
+        step.ensure(r, "hipMalloc rows");
+        trans.ensure(r, "hipMalloc rows");
+        job.ensure(r, "hipMalloc rows");
+        chan.ensure(r, "hipMalloc rows");
+        sender.ensure(r, "hipMalloc rows");
+        status.ensure(r, "hipMalloc rows");
+        digest.ensure(r, "hipMalloc rows");
+        run.ensure(r, "hipMalloc rows");
+        if (feats) X.ensure(r * fpad, "hipMalloc features");
+    }
+    IsRows view() const {
+        return IsRows{loc.p, vars.p, t.p, log2lr.p, qmax.p, qmin.p, step.p, trans.p, chan.p, sender.p,
+                      status.p, digest.p, run.p, job.p};
+    }
+};
+
+// Per-run results of a batch, by job.
+struct OutSet {
+    DevBuf<uint8_t> status;
+    DevBuf<double> log2lr, score, t;
+    DevBuf<uint32_t> step, trans;
+    DevBuf<uint64_t> digest;
+    void ensure(size_t n) {
+        status.ensure(n, "hipMalloc results");
+        for (DevBuf<double> *b : {&log2lr, &score, &t}) b->ensure(n, "hipMalloc results");
+        step.ensure(n, "hipMalloc results");
+        trans.ensure(n, "hipMalloc results");
+        digest.ensure(n, "hipMalloc results");
+    }
+    IsOut view() const { return IsOut{status.p, log2lr.p, score.p, t.p, step.p, trans.p, digest.p}; }
+};
+
 constexpr uint64_t kTrainRunBase = 1ull << 48;  // run indices of training rounds
 constexpr int kMaxBatchRows = 1 << 18;
-constexpr int kTrainChunk = 1 << 16;
+constexpr int kTrainChunk = 1 << 21;  // pairs per training pass (all of a round's, usually)
 constexpr double kZ95 = 1.959963984540054;
 
 }  // namespace
@@ -104,15 +151,13 @@ struct smc_is {
     std::vector<int32_t> h_group;
     DevBuf<float> var_scale, theta_fixed;
     DevBuf<uint16_t> feat0;
-    // rows of a lock-step batch
-    DevBuf<uint16_t> loc;
-    DevBuf<double> vars, t, log2lr, qmax, qmin;
-    DevBuf<uint32_t> step, trans;
-    DevBuf<int32_t> chan, sender, pos;
-    DevBuf<uint8_t> status;
-    DevBuf<uint64_t> digest, run, runs_in;
+    // rows of a lock-step batch, their results, per-row scratch
+    RowSet rows_a, rows_b;
+    OutSet out;
+    DevBuf<int32_t> pos, cpos;
+    DevBuf<uint64_t> runs_in;
     DevBuf<float> weight;
-    DevBuf<uint16_t> X[2], h1, h2;
+    DevBuf<uint16_t> h1, h2;
     DevBuf<float> theta, thg;
     // recorded (run, macro-step) pairs
     DevBuf<uint16_t> rX;
@@ -121,9 +166,10 @@ struct smc_is {
     // training activations of one chunk of pairs
     DevBuf<uint16_t> th1, th2, Gb, dh2b, dh1b, xin;
     DevBuf<float> tth, tthg, dh2, dh1, work, xf;
-    DevBuf<unsigned int> counters;  // [0] active rows, [1] record slots used
+    DevBuf<unsigned int> counters;  // [0] active rows, [1] record slots used, [2] compaction count
     unsigned int *h_counters = nullptr;
     int adam_t = 0;
+    bool zero_output = true;  // output layer all zero: theta == 0 exactly
     bool trained = false;
 
     ~smc_is() {
@@ -135,22 +181,13 @@ struct smc_is {
 
     void ensure_rows(int rows, bool feats) {
         const size_t r = (size_t)rows;
-        loc.ensure(r * n_comp, "hipMalloc rows");
-        vars.ensure(r * std::max(n_vars, 1), "hipMalloc rows");
-        for (DevBuf<double> *b : {&t, &log2lr, &qmax, &qmin}) b->ensure(r, "hipMalloc rows");
-        step.ensure(r, "hipMalloc rows");
-        trans.ensure(r, "hipMalloc rows");
-        chan.ensure(r, "hipMalloc rows");
-        sender.ensure(r, "hipMalloc rows");
+        rows_a.ensure(r, n_comp, n_vars, fpad, feats);
+        out.ensure(r);
         pos.ensure(r, "hipMalloc rows");
-        status.ensure(r, "hipMalloc rows");
-        digest.ensure(r, "hipMalloc rows");
-        run.ensure(r, "hipMalloc rows");
+        cpos.ensure(r, "hipMalloc rows");
         runs_in.ensure(r, "hipMalloc rows");
         weight.ensure(r, "hipMalloc rows");
         if (feats) {
-            X[0].ensure(r * fpad, "hipMalloc features");
-            X[1].ensure(r * fpad, "hipMalloc features");
             h1.ensure(r * H, "hipMalloc activations");
             h2.ensure(r * H, "hipMalloc activations");
             theta.ensure(r * Lpad, "hipMalloc tilts");
@@ -158,14 +195,15 @@ struct smc_is {
         }
     }
 
-    void ensure_train(int rows) {
+    // per_location: also the per-location gradient buffer (the MLP test entry)
+    void ensure_train(int rows, bool per_location = false) {
         const size_t r = (size_t)rows;
         th1.ensure(r * H, "hipMalloc training");
         th2.ensure(r * H, "hipMalloc training");
         dh2b.ensure(r * H, "hipMalloc training");
         dh1b.ensure(r * H, "hipMalloc training");
         Gb.ensure(r * Gpad, "hipMalloc training");
-        tth.ensure(r * Lpad, "hipMalloc training");
+        if (per_location) tth.ensure(r * Lpad, "hipMalloc training");
         tthg.ensure(r * Gpad, "hipMalloc training");
         dh2.ensure(r * H, "hipMalloc training");
         dh1.ensure(r * H, "hipMalloc training");
@@ -175,8 +213,7 @@ struct smc_is {
     IsParams params(int rows, uint64_t seed) const {
         IsParams P{};
         P.m = m->dm;
-        P.st = IsRows{loc.p, vars.p, t.p, log2lr.p, qmax.p, qmin.p, step.p, trans.p, chan.p, sender.p,
-                      status.p, digest.p, run.p};
+        P.st = rows_a.view();
         P.n_rows = rows;
         P.lanes = S;
         P.var_stride = stride;
@@ -272,15 +309,17 @@ struct smc_is {
         int n_valid = 0;
     };
 
-    // Run the rows of one batch to their end; returns the number of launches.
+    // Run the rows of one batch to their end; the results land in `out`
+    // by job (the row's index at the start).  Returns the number of launches.
     int run_rows(const Batch &b) {
         hipStream_t s = stream();
-        const int rows = (int)rup((size_t)b.n_valid, 64);
+        int rows = (int)rup((size_t)b.n_valid, 64);
         const bool feats = b.tilt == Tilt::MLP;
         if (feats && !b.lockstep) throw std::runtime_error("the MLP proposal needs lock-step advance");
         ensure_rows(rows, feats);
+        RowSet *A = &rows_a, *B = &rows_b;
         IsParams P = params(rows, b.seed);
-        P.feat = feats ? X[0].p : nullptr;
+        P.feat = feats ? A->X.p : nullptr;
         hip_check(smc::is_init_rows(P, b.d_runs, b.run_begin, b.n_valid, feat0.p, s), "init rows");
         P.early_stop = b.early_stop ? 1 : 0;
         P.trace = b.trace ? 1 : 0;
@@ -290,29 +329,50 @@ struct smc_is {
             P.rec_X = rX.p; P.rec_A = rA.p; P.rec_B = rB.p; P.rec_w = rw.p;
             hip_check(hipMemsetAsync(counters.p + 1, 0, sizeof(unsigned int), s), "clear record count");
         }
-        int cur = 0, launches = 0;
+        int launches = 0;
         for (;;) {
             if (feats) {
-                forward(X[cur].p, rows, h1.p, h2.p, thg.p, theta.p);
+                // the tilts from the features of the rows' states (the
+                // advance kernel then overwrites the features, in order)
+                forward(A->X.p, rows, h1.p, h2.p, thg.p, theta.p);
                 P.theta = theta.p;
                 P.theta_stride = Lpad;
-                P.feat = X[cur ^ 1].p;
             }
             if (b.record) {
-                hip_check(smc::is_record_slots(status.p, rows, pos.p, counters.p + 1, (unsigned int)rec_cap, s),
+                hip_check(smc::is_record_slots(A->status.p, rows, pos.p, counters.p + 1, (unsigned int)rec_cap, s),
                           "record slots");
                 P.rec_pos = pos.p;
             }
             hip_check(hipMemsetAsync(counters.p, 0, sizeof(unsigned int), s), "clear active count");
             hip_check(smc::is_launch_advance(P, C, occ, lds, s), "advance kernel launch");
             ++launches;
-            cur ^= 1;
             if (!b.lockstep) break;
             hip_check(hipMemcpyAsync(h_counters, counters.p, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, s),
                       "copy active count");
             hip_check(hipStreamSynchronize(s), "advance");
-            if (h_counters[0] == 0) break;
+            const int active = (int)h_counters[0];
+            if (active == 0) break;
+            if (active * 2 <= rows && rows > 1024) {
+                // compaction: results of the finished rows out, live rows
+                // to the front of the other set, in row order
+                hip_check(smc::is_collect(P, out.view(), s), "collect results");
+                hip_check(hipMemsetAsync(counters.p + 2, 0, sizeof(unsigned int), s), "clear compaction count");
+                hip_check(smc::is_record_slots(A->status.p, rows, cpos.p, counters.p + 2, 0xFFFFFFFFu, s),
+                          "compaction slots");
+                const int nrows = (int)rup((size_t)active, 64);
+                B->ensure((size_t)nrows, n_comp, n_vars, fpad, feats);
+                hip_check(hipMemsetAsync(B->status.p, 0, (size_t)nrows, s), "clear rows");
+                hip_check(hipMemsetAsync(B->job.p, 0xFF, (size_t)nrows * 4, s), "clear rows");
+                hip_check(smc::is_compact(P, cpos.p, B->view(), feats ? A->X.p : nullptr, feats ? B->X.p : nullptr,
+                                          s), "compact rows");
+                std::swap(A, B);
+                rows = nrows;
+                P.st = A->view();
+                P.n_rows = rows;
+                P.feat = feats ? A->X.p : nullptr;
+            }
         }
+        hip_check(smc::is_collect(P, out.view(), s), "collect results");
         hip_check(hipStreamSynchronize(s), "advance");
         return launches;
     }
@@ -330,7 +390,6 @@ struct smc_is {
         rw.ensure(rec_cap, "hipMalloc records");
         rAg.ensure(rec_cap * Gpad, "hipMalloc records");
         rBg.ensure(rec_cap * Gpad, "hipMalloc records");
-        ensure_train(kTrainChunk);
     }
 
     // Replay runs (device array runs_in, weights in `weight`) under the
@@ -368,6 +427,7 @@ struct smc_is {
     void ce_gradient() {
         hipStream_t st = stream();
         hip_check(hipMemsetAsync(g.p, 0, n_params * 4, st), "clear gradient");
+        if (rec_pairs) ensure_train((int)std::min<size_t>(kTrainChunk, rec_pairs));
         for (size_t off = 0; off < rec_pairs; off += kTrainChunk) {
             const int rows_c = (int)std::min<size_t>(kTrainChunk, rec_pairs - off);
             const uint16_t *Xc = rX.p + off * fpad;
@@ -537,7 +597,7 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         s->var_scale.ensure(scale.size(), "hipMalloc scales");
         s->theta_fixed.ensure(L, "hipMalloc tilts");
         s->feat0.ensure(fp, "hipMalloc features");
-        s->counters.ensure(2, "hipMalloc counters");
+        s->counters.ensure(3, "hipMalloc counters");
         hip_check(hipHostMalloc((void **)&s->h_counters, 2 * sizeof(unsigned int), hipHostMallocDefault), "hipHostMalloc");
         hip_check(hipMemcpyAsync(s->tiltable.p, s->h_tiltable.data(), L, hipMemcpyHostToDevice, st), "upload");
         hip_check(hipMemcpyAsync(s->var_scale.p, scale.data(), scale.size() * 4, hipMemcpyHostToDevice, st), "upload");
@@ -601,6 +661,7 @@ int smc_is_set_params(smc_is *s, const float *in) {
     return guarded([&]() -> int {
         hip_check(hipSetDevice(s->m->device), "hipSetDevice");
         s->upload_weights(std::vector<float>(in, in + s->n_params));
+        s->zero_output = false;
         return SMC_OK;
     });
 }
@@ -612,7 +673,7 @@ int smc_is_mlp_eval(smc_is *s, const float *x, int rows, const float *gout, floa
         hipStream_t st = s->stream();
         const int rp = (int)rup((size_t)rows, 64);
         const size_t nx = (size_t)rp * s->fpad, nt = (size_t)rp * s->Lpad;
-        s->ensure_train(rp);
+        s->ensure_train(rp, true);
         s->xf.ensure(std::max(nx, nt), "hipMalloc inputs");
         s->xin.ensure(nx, "hipMalloc inputs");
         hip_check(hipMemsetAsync(s->xf.p, 0, nx * 4, st), "clear inputs");
@@ -702,12 +763,12 @@ int smc_is_sample(smc_is *s, const float *theta, int use_mlp, uint64_t seed, uin
             b.run_begin = run_begin + done;
             b.n_valid = nb;
             s->run_rows(b);
-            s->fetch(stat, s->status, nb);
-            s->fetch(lr, s->log2lr, nb);
-            s->fetch(stp, s->step, nb);
-            s->fetch(trn, s->trans, nb);
-            if (out && out->digest) s->fetch(dig, s->digest, nb);
-            if (out && out->time) s->fetch(tm, s->t, nb);
+            s->fetch(stat, s->out.status, nb);
+            s->fetch(lr, s->out.log2lr, nb);
+            s->fetch(stp, s->out.step, nb);
+            s->fetch(trn, s->out.trans, nb);
+            if (out && out->digest) s->fetch(dig, s->out.digest, nb);
+            if (out && out->time) s->fetch(tm, s->out.t, nb);
             hip_check(hipStreamSynchronize(st), "copy rows");
             accumulate(stat, lr, stp, trn, nb, &sw, &sw2, &hits, &steps, &trans);
             if (out) {
@@ -743,9 +804,12 @@ int smc_is_train(smc_is *s, smc_is_round *rounds) {
         std::vector<uint32_t> stp, trn;
         for (int r = 0; r < c.rounds; ++r) {
             auto t0 = std::chrono::steady_clock::now();
-            // 1. sample a round under the current proposal (each round its own runs)
+            // 1. sample a round under the current proposal (each round its own
+            // runs); before any training the output layer is zero, theta is
+            // exactly 0, and the runs are the base law's in one launch
             smc_is::Batch b;
-            b.tilt = Tilt::MLP;
+            b.tilt = s->zero_output ? Tilt::NONE : Tilt::MLP;
+            b.lockstep = !s->zero_output;
             b.seed = c.seed;
             b.run_begin = kTrainRunBase + (uint64_t)r * c.runs_per_round;
             std::vector<uint64_t> run_of;
@@ -759,11 +823,11 @@ int smc_is_train(smc_is *s, smc_is_round *rounds) {
                 b.n_valid = nb;
                 b.run_begin = kTrainRunBase + (uint64_t)r * c.runs_per_round + (uint64_t)done;
                 s->run_rows(b);
-                s->fetch(stat, s->status, nb);
-                s->fetch(lr, s->log2lr, nb);
-                s->fetch(qm, s->q.is_max ? s->qmax : s->qmin, nb);
-                s->fetch(stp, s->step, nb);
-                s->fetch(trn, s->trans, nb);
+                s->fetch(stat, s->out.status, nb);
+                s->fetch(lr, s->out.log2lr, nb);
+                s->fetch(qm, s->out.score, nb);
+                s->fetch(stp, s->out.step, nb);
+                s->fetch(trn, s->out.trans, nb);
                 hip_check(hipStreamSynchronize(st), "copy rows");
                 accumulate(stat, lr, stp, trn, nb, &sw, &sw2, &hits, &steps, &trans);
                 for (int i = 0; i < nb; ++i) {
@@ -826,7 +890,7 @@ int smc_is_train(smc_is *s, smc_is_round *rounds) {
                 hip_check(hipMemcpyAsync(s->weight.p, ew.data(), rows * 4, hipMemcpyHostToDevice, st), "upload weights");
                 // 4. replay the elites under the same proposal, recording pairs
                 smc_is::Batch rb;
-                rb.tilt = Tilt::MLP;
+                rb.tilt = s->zero_output ? Tilt::NONE : Tilt::MLP;
                 rb.seed = c.seed;
                 rb.n_valid = ne;
                 info.pairs = s->record(rb);
@@ -835,6 +899,7 @@ int smc_is_train(smc_is *s, smc_is_round *rounds) {
                     s->ce_gradient();
                     hip_check(smc::adam_step(s->w.p, s->g.p, s->m1.p, s->m2.p, s->n_params, c.learning_rate,
                                              ++s->adam_t, s->wb.p, st), "Adam step");
+                    s->zero_output = false;
                 }
                 hip_check(hipStreamSynchronize(st), "training");
             }
@@ -916,10 +981,10 @@ int smc_is_estimate(smc_is *s, uint64_t seed, uint64_t run_begin, uint64_t batch
             b.run_begin = run_begin + done;
             b.n_valid = nb;
             s->run_rows(b);
-            s->fetch(stat, s->status, nb);
-            s->fetch(lr, s->log2lr, nb);
-            s->fetch(stp, s->step, nb);
-            s->fetch(trn, s->trans, nb);
+            s->fetch(stat, s->out.status, nb);
+            s->fetch(lr, s->out.log2lr, nb);
+            s->fetch(stp, s->out.step, nb);
+            s->fetch(trn, s->out.trans, nb);
             hip_check(hipStreamSynchronize(s->stream()), "copy rows");
             accumulate(stat, lr, stp, trn, nb, &sw, &sw2, &hits, &steps, &trans);
             done += (uint64_t)nb;

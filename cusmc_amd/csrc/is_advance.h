@@ -98,13 +98,7 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
     const double LN2 = 0.69314718055994530942;
 
     uint8_t stat = st.status[row];
-    if (!(stat & IS_ACTIVE)) {
-        if (P.feat) {
-            uint16_t *f = P.feat + (size_t)row * P.fpad;
-            for (int k = sl; k < P.fpad; k += S) f[k] = 0;
-        }
-        return;
-    }
+    if (!(stat & IS_ACTIVE)) return;  // its features and tilts are not read
 
     // load the run
     const uint64_t run = st.run[row];
@@ -134,7 +128,7 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
     const int rec = P.rec_pos ? P.rec_pos[row] : -1;
     if (rec >= 0) {
         write_features<C>(P, P.rec_X + (size_t)rec * P.fpad, loc, vars, t, sl, S, n);
-        if (sl == 0) P.rec_w[rec] = P.weight[row];
+        if (sl == 0) P.rec_w[rec] = P.weight[st.job[row]];
     }
     bool first_race = true;
     bool hit = false, end = false, fired_any = false;

@@ -23,6 +23,7 @@ __global__ __launch_bounds__(64) void is_init_kernel(IsParams P, const uint64_t 
     if (tid == 0) {
         const uint64_t run = valid ? (runs ? runs[row] : run_begin + (uint64_t)row) : 0;
         st.run[row] = run;
+        st.job[row] = valid ? (uint32_t)row : 0xFFFFFFFFu;
         st.t[row] = 0.0;
         st.log2lr[row] = 0.0;
         st.qmax[row] = P.qinit;
@@ -33,6 +34,52 @@ __global__ __launch_bounds__(64) void is_init_kernel(IsParams P, const uint64_t 
         st.sender[row] = -1;
         st.digest[row] = smc_mix(0, run);
         st.status[row] = valid ? (uint8_t)IS_ACTIVE : (uint8_t)0;
+    }
+}
+
+__global__ void is_collect_kernel(IsParams P, IsOut out) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= P.n_rows) return;
+    const IsRows &st = P.st;
+    const uint8_t s = st.status[row];
+    if (s & IS_ACTIVE) return;
+    const uint32_t j = st.job[row];
+    if (j == 0xFFFFFFFFu) return;  // padding
+    out.status[j] = s;
+    out.log2lr[j] = st.log2lr[row];
+    out.score[j] = P.is_max ? st.qmax[row] : st.qmin[row];
+    out.t[j] = st.t[row];
+    out.step[j] = st.step[row];
+    out.trans[j] = st.trans[row];
+    out.digest[j] = st.digest[row];
+}
+
+// One workgroup per source row; active rows move to pos[row].
+__global__ __launch_bounds__(64) void is_compact_kernel(IsParams P, const int32_t *pos, IsRows dst,
+                                                        const uint16_t *feat, uint16_t *dst_feat) {
+    const int row = blockIdx.x;
+    const int p = pos[row];
+    if (p < 0) return;
+    const int tid = threadIdx.x;
+    const IsRows &st = P.st;
+    const int n = P.m.n_comp, nv = P.m.n_vars;
+    for (int c = tid; c < n; c += 64) dst.loc[(size_t)p * n + c] = st.loc[(size_t)row * n + c];
+    for (int v = tid; v < nv; v += 64) dst.vars[(size_t)p * nv + v] = st.vars[(size_t)row * nv + v];
+    if (feat)
+        for (int k = tid; k < P.fpad; k += 64) dst_feat[(size_t)p * P.fpad + k] = feat[(size_t)row * P.fpad + k];
+    if (tid == 0) {
+        dst.t[p] = st.t[row];
+        dst.log2lr[p] = st.log2lr[row];
+        dst.qmax[p] = st.qmax[row];
+        dst.qmin[p] = st.qmin[row];
+        dst.step[p] = st.step[row];
+        dst.trans[p] = st.trans[row];
+        dst.chan[p] = st.chan[row];
+        dst.sender[p] = st.sender[row];
+        dst.status[p] = st.status[row];
+        dst.digest[p] = st.digest[row];
+        dst.run[p] = st.run[row];
+        dst.job[p] = st.job[row];
     }
 }
 
@@ -98,18 +145,29 @@ __global__ void expand_groups_kernel(const float *tg, int gpad, const int32_t *g
 }
 
 // Gradient of a group's output: the sum over its member locations, in the
-// fixed member order (CSR member_first / members).
+// fixed member order (CSR member_first / members); one thread per (row,
+// group).  The padding outputs are cleared by the launcher.
 __global__ void group_sum_kernel(const float *gl, int lpad, const int32_t *member_first, const int32_t *members,
                                  int n_groups, int gpad, size_t n, float *gg, uint16_t *ggb) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int gidx = (int)(i % (size_t)gpad);
-    const size_t row = i / (size_t)gpad;
+    const int gidx = (int)(i % (size_t)n_groups);
+    const size_t row = i / (size_t)n_groups;
+    const float *src = gl + row * lpad;
     float s = 0.f;
-    if (gidx < n_groups)
-        for (int k = member_first[gidx]; k < member_first[gidx + 1]; ++k) s += gl[row * lpad + members[k]];
-    gg[i] = s;
-    if (ggb) ggb[i] = f32_to_bf16(s);
+    for (int k = member_first[gidx]; k < member_first[gidx + 1]; ++k) s += src[members[k]];
+    gg[row * gpad + gidx] = s;
+    if (ggb) ggb[row * gpad + gidx] = f32_to_bf16(s);
+}
+
+__global__ void clear_columns_kernel(float *gg, uint16_t *ggb, int from, int gpad, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = gpad - from;
+    if (i >= n * (size_t)w) return;
+    const size_t row = i / (size_t)w;
+    const int col = from + (int)(i % (size_t)w);
+    gg[row * gpad + col] = 0.f;
+    if (ggb) ggb[row * gpad + col] = 0;
 }
 
 // Column sums over a slice of rows (blockIdx.y), 64 columns per workgroup.
@@ -325,10 +383,16 @@ hipError_t expand_groups(const float *tg, int gpad, const int32_t *group, int n_
 
 hipError_t group_sum(const float *gl, int lpad, const int32_t *member_first, const int32_t *members, int n_groups,
                      int gpad, int rows, float *gg, uint16_t *ggb, hipStream_t s) {
-    const size_t n = (size_t)rows * gpad;
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(group_sum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, gl, lpad, member_first,
-                       members, n_groups, gpad, n, gg, ggb);
+    if (rows <= 0) return hipSuccess;
+    const size_t n = (size_t)rows * n_groups;
+    if (n)
+        hipLaunchKernelGGL(group_sum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, gl, lpad, member_first,
+                           members, n_groups, gpad, n, gg, ggb);
+    if (gpad > n_groups) {
+        const size_t z = (size_t)rows * (gpad - n_groups);
+        hipLaunchKernelGGL(clear_columns_kernel, dim3((unsigned)((z + 255) / 256)), dim3(256), 0, s, gg, ggb, n_groups,
+                           gpad, (size_t)rows);
+    }
     return hipGetLastError();
 }
 
@@ -336,6 +400,19 @@ hipError_t is_init_rows(const IsParams &p, const uint64_t *runs, uint64_t run_be
                         const uint16_t *feat0, hipStream_t s) {
     if (p.n_rows <= 0) return hipSuccess;
     hipLaunchKernelGGL(is_init_kernel, dim3(p.n_rows), dim3(64), 0, s, p, runs, run_begin, n_valid, feat0);
+    return hipGetLastError();
+}
+
+hipError_t is_collect(const IsParams &p, const IsOut &out, hipStream_t s) {
+    if (p.n_rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(is_collect_kernel, dim3((unsigned)((p.n_rows + 255) / 256)), dim3(256), 0, s, p, out);
+    return hipGetLastError();
+}
+
+hipError_t is_compact(const IsParams &p, const int32_t *pos, const IsRows &dst, const uint16_t *feat,
+                      uint16_t *dst_feat, hipStream_t s) {
+    if (p.n_rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(is_compact_kernel, dim3(p.n_rows), dim3(64), 0, s, p, pos, dst, feat, dst_feat);
     return hipGetLastError();
 }
 

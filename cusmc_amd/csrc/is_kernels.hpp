@@ -17,6 +17,15 @@ struct IsRows {
     uint8_t *status;      // IS_* bits
     uint64_t *digest;
     uint64_t *run;        // global run index (Philox counter)
+    uint32_t *job;        // index of the run in the batch (outputs, weights)
+};
+
+// Results of a batch's runs, indexed by job.
+struct IsOut {
+    uint8_t *status;
+    double *log2lr, *score, *t;   // score: the running max (min) of the query variable
+    uint32_t *step, *trans;
+    uint64_t *digest;
 };
 
 struct IsParams {
@@ -47,7 +56,7 @@ struct IsParams {
     float *rec_A;               // [slots][lpad] exponential wins per location
     float *rec_B;               // [slots][lpad] ln2 * dt * base rate per location
     float *rec_w;               // [slots] weight of the row
-    const float *weight;        // [rows]
+    const float *weight;        // [jobs]
     unsigned int *active_count; // rows still active after the launch
 };
 
@@ -88,6 +97,13 @@ hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, 
 // rows >= n_valid are inactive.  feat0 is the feature row of the initial state.
 hipError_t is_init_rows(const IsParams &p, const uint64_t *runs, uint64_t run_begin, int n_valid,
                         const uint16_t *feat0, hipStream_t s);
+// Results of the rows that are no longer active, into out[job].
+hipError_t is_collect(const IsParams &p, const IsOut &out, hipStream_t s);
+// Stable compaction of the active rows of p.st (and their feature rows in
+// feat) into dst / dst_feat, at the positions pos (a record-slot scan with
+// base 0).
+hipError_t is_compact(const IsParams &p, const int32_t *pos, const IsRows &dst, const uint16_t *feat,
+                      uint16_t *dst_feat, hipStream_t s);
 // Record slots for the active rows, in row order, after *base; advances *base.
 hipError_t is_record_slots(const uint8_t *status, int n_rows, int32_t *pos, unsigned int *base, unsigned int cap,
                            hipStream_t s);
