@@ -174,11 +174,12 @@ def bench_is(args) -> dict:
     sim.autotune(q, seed=SEED + 999, sample_runs=65536)
     sim.run(q, seed=SEED + 998, n_runs=1 << 20)  # warm
     # plain: batches of runs until the CLT half-width is within 10%, capped
+    # (a cap of 0 skips it: the learned sampler alone)
     t0 = time.perf_counter()
     hits = n = 0
     batch = args.runs or 8_000_000
     rel = math.inf
-    while time.perf_counter() - t0 < args.plain_cap_s:
+    while args.plain_cap_s > 0 and time.perf_counter() - t0 < args.plain_cap_s:
         r = sim.run(q, seed=SEED, n_runs=batch, run_begin=n)
         hits += r.hits
         n += r.n_runs
@@ -188,11 +189,11 @@ def bench_is(args) -> dict:
         if hits >= 10 and rel <= 0.1:
             break
     plain_s = time.perf_counter() - t0
-    plain_p = hits / n
-    plain_rate = n / plain_s
+    plain_p = hits / n if n else math.nan
+    plain_rate = n / plain_s if n else math.nan
     # runs plain needs for +-10% at the exact p, and the time at its rate
     need = (1.959963984540054 / 0.1) ** 2 * (1 - exact) / exact
-    plain_to_ci_s = plain_s if rel <= 0.1 else need / plain_rate
+    plain_to_ci_s = plain_s if rel <= 0.1 else (need / plain_rate if n else math.nan)
     # learned IS: create, train (5 rounds x 1e5), estimate to +-10%
     t1 = time.perf_counter()
     s = cs.ImportanceSampler(sim, q, rounds=5, runs_per_round=100_000, epochs=args.epochs,
@@ -207,7 +208,7 @@ def bench_is(args) -> dict:
     plain_half = 1.959963984540054 * math.sqrt(plain_p * (1 - plain_p) / n) if hits else math.inf
     return {
         "metric": "time to +-10% relative half-width at 95%, learned IS vs plain (speed-up)",
-        "value": plain_to_ci_s / is_s, "unit": "x", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "value": plain_to_ci_s / is_s if n else None, "unit": "x", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": is_s * 1e3, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64 (simulation), bf16/fp32 (MLP)",
         "data": "synthetic (SIR N=100 network generated from the seed)",

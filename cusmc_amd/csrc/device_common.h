@@ -341,11 +341,68 @@ __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int
     return true;
 }
 
+// Rotate slot arrays by one (slot 1 to the front): a rolled loop over the
+// slots then touches slot 0 only, a constant index, so the arrays stay in
+// registers rather than being spilled for dynamic indexing.
+template <int C, class T>
+__device__ __forceinline__ void rotate_slots(T (&a)[C]) {
+    const T f = a[0];
+#pragma unroll
+    for (int k = 0; k < C - 1; ++k) a[k] = a[k + 1];
+    a[C - 1] = f;
+}
+
+// Receivers of one broadcast fired one at a time in component order
+// (SyncInSerial simulation.cu:378-404): the lanes holding enabled receivers
+// in lane order, their slots in slot order.  Enabled sets mk were computed
+// before any of them fires; each picks with its own stream.
+template <int C>
+__device__ __forceinline__ void fire_receivers_serial(uint64_t (&mk)[C], int (&loc)[C], bool any, double *vars,
+                                                      const DevModel &m, LaneQuery &q, bool &goal, uint64_t seed,
+                                                      uint64_t run, uint32_t step, int lane, int seg_base,
+                                                      uint64_t seg_mask, uint32_t &trans, bool trace,
+                                                      uint64_t &digest) {
+    int dummy_chan = 0;
+    for (uint64_t lanes_with = __ballot(any) & seg_mask; lanes_with; lanes_with &= lanes_with - 1) {
+        const int L = lowest_lane(lanes_with);
+        // slot j is at the front of the rotated slot arrays
+#pragma clang loop unroll(disable)
+        for (int j = 0; j < C; ++j) {
+            if ((__ballot(mk[0] != 0) >> L) & 1ull) {
+                const int c = (L - seg_base) * C + j;
+                int e_fired = -1;
+                if (lane == L) {
+                    const int first = (int)m.locs[loc[0]].edge_first;
+                    e_fired = first + pick_edge(mk[0], first, seed, run, step, SMC_P_RECEIVER, (uint32_t)c, vars, m);
+                    loc[0] = fire_edge(e_fired, vars, m, q, goal, dummy_chan);
+                }
+                wave_sync();
+                ++trans;
+                if (trace) {
+                    const int e = __shfl(e_fired, L);
+                    digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)c << 20) | (uint64_t)e);
+                }
+            }
+            rotate_slots<C>(mk);
+            rotate_slots<C>(loc);
+        }
+    }
+}
+
+// Register-resident slot arrays read at a run-time slot index.  Each element
+// passes through an empty asm so the select chain stays a chain of register
+// selects: without it the optimiser folds select(load a[k], load a[j]) into
+// a load through a selected address, which pins the array in scratch.
+template <class T>
+__device__ __forceinline__ T in_register(T v) {
+    asm("" : "+v"(v));
+    return v;
+}
 template <int C, class T>
 __device__ __forceinline__ T slot_get(const T (&a)[C], int j) {
-    T r = a[0];
+    T r = in_register(a[0]);
 #pragma unroll
-    for (int k = 1; k < C; ++k) r = (j == k) ? a[k] : r;
+    for (int k = 1; k < C; ++k) r = (j == k) ? in_register(a[k]) : r;
     return r;
 }
 template <int C, class T>

@@ -155,32 +155,10 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                         mk[j] = (c < n && c != sender) ? enabled_mask(loc[j], chan, vars, m) : 0ull;
                         any |= mk[j] != 0;
                     }
-                    int dummy = 0;
-                    const bool fast = fire_receivers_fast<C>(mk, loc, vars, m, q, goal_local, P.seed, run, step, sl,
-                                                             S, seg_base, seg_mask, trans, P.trace != 0, digest);
-                    for (uint64_t lw = fast ? 0ull : (__ballot(any) & seg_mask); lw; lw &= lw - 1) {
-                        const int L = lowest_lane(lw);
-#pragma clang loop unroll(disable)
-                        for (int j = 0; j < C; ++j) {
-                            if (!((__ballot(slot_get<C>(mk, j) != 0) >> L) & 1ull)) continue;
-                            const int c = (L - seg_base) * C + j;
-                            int e_fired = -1;
-                            if (lane == L) {
-                                const int lj = slot_get<C>(loc, j);
-                                const int first = (int)m.locs[lj].edge_first;
-                                const int i = pick_edge(slot_get<C>(mk, j), first, P.seed, run, step, SMC_P_RECEIVER,
-                                                        (uint32_t)c, vars, m);
-                                e_fired = first + i;
-                                slot_set<C>(loc, j, fire_edge(e_fired, vars, m, q, goal_local, dummy));
-                            }
-                            wave_sync();
-                            ++trans;
-                            if (P.trace) {
-                                const int e = __shfl(e_fired, L);
-                                digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)c << 20) | (uint64_t)e);
-                            }
-                        }
-                    }
+                    if (!fire_receivers_fast<C>(mk, loc, vars, m, q, goal_local, P.seed, run, step, sl, S, seg_base,
+                                                seg_mask, trans, P.trace != 0, digest))
+                        fire_receivers_serial<C>(mk, loc, any, vars, m, q, goal_local, P.seed, run, step, lane,
+                                                 seg_base, seg_mask, trans, P.trace != 0, digest);
                     chan = 0;
                     sender = -1;
                     goal = (__ballot(goal_local) & seg_mask) != 0;
@@ -402,33 +380,10 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                 mk[j] = (c < n && c != sender) ? enabled_mask(loc[j], chan, vars, m) : 0ull;
                 any |= mk[j] != 0;
             }
-            int dummy_chan = 0;
-            const bool fast = fire_receivers_fast<C>(mk, loc, vars, m, q, goal_local, P.seed, run, step, sl, S,
-                                                     seg_base, seg_mask, trans, P.trace != 0, digest);
-            for (uint64_t lanes_with = fast ? 0ull : (__ballot(any) & seg_mask); lanes_with;
-                 lanes_with &= lanes_with - 1) {
-                const int L = lowest_lane(lanes_with);
-#pragma clang loop unroll(disable)
-                for (int j = 0; j < C; ++j) {
-                    if (!((__ballot(slot_get<C>(mk, j) != 0) >> L) & 1ull)) continue;
-                    const int c = (L - seg_base) * C + j;
-                    int e_fired = -1;
-                    if (lane == L) {
-                        const int lj = slot_get<C>(loc, j);
-                        const int first = (int)m.locs[lj].edge_first;
-                        const int i = pick_edge(slot_get<C>(mk, j), first, P.seed, run, step, SMC_P_RECEIVER,
-                                                (uint32_t)c, vars, m);
-                        e_fired = first + i;
-                        slot_set<C>(loc, j, fire_edge(e_fired, vars, m, q, goal_local, dummy_chan));
-                    }
-                    wave_sync();
-                    ++trans;
-                    if (P.trace) {
-                        const int e = __shfl(e_fired, L);
-                        digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)c << 20) | (uint64_t)e);
-                    }
-                }
-            }
+            if (!fire_receivers_fast<C>(mk, loc, vars, m, q, goal_local, P.seed, run, step, sl, S, seg_base, seg_mask,
+                                        trans, P.trace != 0, digest))
+                fire_receivers_serial<C>(mk, loc, any, vars, m, q, goal_local, P.seed, run, step, lane, seg_base,
+                                         seg_mask, trans, P.trace != 0, digest);
             chan = 0;
             sender = -1;
             goal = (__ballot(goal_local) & seg_mask) != 0;
