@@ -66,15 +66,20 @@ def bench_plain(args) -> dict:
     net, q, runs, name = _models(args.workload)
     runs = args.runs or runs
     sim = cs.GpuSimulator(net, device=0)
-    tuned = sim.autotune(q, seed=SEED + 999, sample_runs=max(1024, min(runs // 4, 16384)))
-    _progress(f"autotune: {tuned.lanes_per_run} lanes per run")
+    if args.lanes:
+        # a fixed launch shape (profiles pin one kernel build this way)
+        shape = {"lanes_per_run": args.lanes, "occupancy": args.occupancy}
+        policy = args.occupancy
+    else:
+        tuned = sim.autotune(q, seed=SEED + 999, sample_runs=max(1024, min(runs // 4, 16384)))
+        shape, policy = {}, tuned.occupancy
     for w in range(args.warmup):
-        sim.run(q, seed=SEED + 1000 + w, n_runs=runs)
+        sim.run(q, seed=SEED + 1000 + w, n_runs=runs, **shape)
         _progress(f"warmup {w + 1}/{args.warmup}")
     results = []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        results.append(sim.run(q, seed=SEED + k, n_runs=runs))
+        results.append(sim.run(q, seed=SEED + k, n_runs=runs, **shape))
         _progress(f"step {k + 1}/{args.steps}: {results[-1].kernel_ms:.1f} ms kernel")
     wall = time.perf_counter() - t0
     sim.close()
@@ -97,7 +102,8 @@ def bench_plain(args) -> dict:
         "dtype": "f64",
         "data": "synthetic (seeded network generated as UPPAAL XML)",
         "config": {"workload": name, "runs_per_step": runs, "lanes_per_run": results[-1].lanes_per_run,
-                   "waves": results[-1].waves, "tuned_lanes_per_run": tuned.lanes_per_run},
+                   "waves": results[-1].waves, "occupancy_policy": policy,
+                   "launch": "fixed" if args.lanes else "autotuned"},
         "transitions_per_sec": trans / wall,
         "race_steps_per_sec": steps / wall,
         "kernel_runs_per_sec": n / (kms / 1e3),
@@ -302,6 +308,9 @@ def main() -> int:
     ap.add_argument("--plain-cap-s", type=float, default=120.0, help="is: cap on the plain run's wall time")
     ap.add_argument("--epochs", type=int, default=6, help="is: epochs per cross-entropy round")
     ap.add_argument("--lr", type=float, default=1e-2, help="is: Adam step")
+    ap.add_argument("--lanes", type=int, default=0, help="fixed lanes per run (default: autotuned)")
+    ap.add_argument("--occupancy", type=int, default=0, choices=[0, 2],
+                    help="with --lanes: register policy (0 unrestricted, 2 two waves per SIMD)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

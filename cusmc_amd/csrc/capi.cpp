@@ -226,17 +226,17 @@ int smc_model_create(const smc_network *net, int device, smc_model **out) {
         hip_check(hipMemcpy(m->arena, host.data(), total, hipMemcpyHostToDevice), "upload model");
         auto at = [&](int i) { return (const void *)((const uint8_t *)m->arena + parts[i].off); };
         DevModel &d = m->dm;
-        d.locs = (const DevLoc *)at(0);
-        d.invs = (const DevInv *)at(1);
-        d.edges = (const DevEdge *)at(2);
-        d.guards = (const DevGuard *)at(3);
-        d.upds = (const DevUpd *)at(4);
-        d.consts = (const double *)at(5);
-        d.code = (const DevInsn *)at(6);
-        d.var_init = (const double *)at(7);
-        d.comp_init_loc = (const int32_t *)at(8);
-        d.clocks = (const uint16_t *)at(9);
-        d.comp_first_loc = (const int32_t *)at(10);
+        d.locs = (decltype(d.locs))at(0);
+        d.invs = (decltype(d.invs))at(1);
+        d.edges = (decltype(d.edges))at(2);
+        d.guards = (decltype(d.guards))at(3);
+        d.upds = (decltype(d.upds))at(4);
+        d.consts = (decltype(d.consts))at(5);
+        d.code = (decltype(d.code))at(6);
+        d.var_init = (decltype(d.var_init))at(7);
+        d.comp_init_loc = (decltype(d.comp_init_loc))at(8);
+        d.clocks = (decltype(d.clocks))at(9);
+        d.comp_first_loc = (decltype(d.comp_first_loc))at(10);
         d.n_comp = im.n_comp;
         d.n_vars = im.n_vars;
         int nclk = 0;

@@ -70,17 +70,49 @@ __device__ __forceinline__ double tree_total(const double (&leaf)[C], int S) {
     return s;
 }
 
+// ------------------------------------------------------------ table records
+
+// A whole table record in 16-byte (8-byte) vector loads.  Copying a record
+// field by field costs one narrow load per field used (seven for a
+// location), each one a separate trip to the cache; the empty asm keeps the
+// optimiser from narrowing the vector loads back to the fields used.
+template <class T>
+__device__ __forceinline__ T load_rec(const SMC_GLOBAL T *p) {
+    static_assert(sizeof(T) % 8 == 0, "record size");
+    T r;
+    if constexpr (sizeof(T) % 16 == 0) {
+        const SMC_GLOBAL uint4 *q = (const SMC_GLOBAL uint4 *)p;
+        uint4 v[sizeof(T) / 16];
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) / 16); ++i) {
+            v[i] = q[i];
+            asm("" : "+v"(v[i].x), "+v"(v[i].y), "+v"(v[i].z), "+v"(v[i].w));
+        }
+        __builtin_memcpy(&r, v, sizeof(T));
+    } else {
+        const SMC_GLOBAL uint2 *q = (const SMC_GLOBAL uint2 *)p;
+        uint2 v[sizeof(T) / 8];
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) / 8); ++i) {
+            v[i] = q[i];
+            asm("" : "+v"(v[i].x), "+v"(v[i].y));
+        }
+        __builtin_memcpy(&r, v, sizeof(T));
+    }
+    return r;
+}
+
 // ------------------------------------------------------------ expressions
 
 // Register-bytecode interpreter (expressions.cu:10-119).  Inlined: an
 // out-of-line call costs the whole kernel registers and time even on models
 // that never reach it.
-__device__ __forceinline__ double eval_code(uint32_t ref, const double *vars, const double *consts,
-                                         const DevInsn *code) {
+__device__ __forceinline__ double eval_code(uint32_t ref, const double *vars, const SMC_GLOBAL double *consts,
+                                            const SMC_GLOBAL DevInsn *code) {
     const uint32_t first = ref & 0xFFFFFFu, count = (ref >> 24) & 0x3Fu;
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
     for (uint32_t k = 0; k < count; ++k) {
-        const DevInsn in = code[first + k];
+        const DevInsn in = load_rec(code + first + k);
         double o[3];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
@@ -124,16 +156,16 @@ __device__ __forceinline__ bool guard_holds(int op, double x, double r) {
 // (check_enabled_edges simulation.cu:268-305, SyncInSerial :363-377).
 __device__ __forceinline__ uint64_t enabled_mask(int loc, int recv_chan, const double *vars,
                                                  const DevModel &m) {
-    const DevLoc &L = m.locs[loc];
+    const DevLoc L = load_rec(m.locs + loc);
     const int ne = L.edge_n;
     uint64_t mask = 0;
     for (int i = 0; i < ne; ++i) {
-        const DevEdge &E = m.edges[L.edge_first + i];
+        const DevEdge E = load_rec(m.edges + L.edge_first + i);
         if (E.never) continue;
         if (recv_chan ? (E.channel != -recv_chan) : (E.channel < 0)) continue;
         bool ok = true;
         for (int g = 0; g < E.guard_n && ok; ++g) {
-            const DevGuard G = m.guards[E.guard_first + g];
+            const DevGuard G = load_rec(m.guards + E.guard_first + g);
             ok = guard_holds(G.op, vars[G.var], eval_ref(G.rhs, vars, m));
         }
         if (ok) mask |= 1ull << i;
@@ -182,7 +214,7 @@ __device__ __forceinline__ double dead_total(uint32_t deadmask, const int (&loc)
     for (int j = 0; j < C; ++j) {
         dl[j] = 0.0;
         if ((deadmask >> j) & 1u) {
-            const DevLoc L = m.locs[loc[j]];
+            const DevLoc L = load_rec(m.locs + loc[j]);
             const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
             dl[j] = r > 0.0 ? r : 0.0;
         }
@@ -199,9 +231,9 @@ struct LaneQuery {
 // (take_transition simulation.cu:88-180).  Returns the destination.
 __device__ __forceinline__ int fire_edge(int e, double *vars, const DevModel &m, LaneQuery &q,
                                          bool &goal, int &chan) {
-    const DevEdge &E = m.edges[e];
+    const DevEdge E = load_rec(m.edges + e);
     for (int k = 0; k < E.upd_n; ++k) {
-        const DevUpd U = m.upds[E.upd_first + k];
+        const DevUpd U = load_rec(m.upds + E.upd_first + k);
         const double v = eval_ref(U.expr, vars, m);
         if ((int)U.var == q.qvar) {
             if (v > q.qmax) q.qmax = v;
@@ -228,7 +260,7 @@ __device__ __forceinline__ bool absorbed(const int (&loc)[C], const double *vars
     for (int j = 0; j < C; ++j) {
         const int c = sl * C + j;
         if (can || c >= n) continue;
-        const DevLoc L = m.locs[loc[j]];
+        const DevLoc L = load_rec(m.locs + loc[j]);
         if (!(L.flags & SMC_LF_TIME_FREE) || L.klass == SMC_K_ZERO) { can = true; continue; }
         if (enabled_mask(loc[j], 0, vars, m) == 0) continue;
         if (L.klass != SMC_K_EXP) { can = true; continue; }
@@ -281,9 +313,9 @@ __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int
             ej[j] = e;
             ++fired;
             any_fired = true;
-            const DevEdge &E = m.edges[e];
+            const DevEdge E = load_rec(m.edges + e);
             for (int k = 0; k < E.upd_n; ++k) {
-                const DevUpd U = m.upds[E.upd_first + k];
+                const DevUpd U = load_rec(m.upds + E.upd_first + k);
                 if ((int)U.var != q.qvar) continue;
                 if (U.kind == SMC_U_ADD) {
                     qsum += U.value;
@@ -315,7 +347,7 @@ __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int
         if (ej[j] >= 0) {
             const DevEdge &E = m.edges[ej[j]];
             for (int k = 0; k < E.upd_n; ++k) {
-                const DevUpd U = m.upds[E.upd_first + k];
+                const DevUpd U = load_rec(m.upds + E.upd_first + k);
                 if (U.kind == SMC_U_ADD) atomicAdd(&vars[U.var], U.value);
                 else vars[U.var] = U.value;
             }

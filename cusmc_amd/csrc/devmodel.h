@@ -115,19 +115,29 @@ struct DevUpd {             // 16 bytes
     double value;           // the added constant (ADD) or the value (SET)
 };
 
+// The tables live in device global memory.  Device code sees their pointers
+// in the global address space, so reads compile to global loads (their own
+// wait counter) rather than flat loads, which the compiler must also order
+// against LDS traffic; host code sees plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SMC_GLOBAL __attribute__((address_space(1)))
+#else
+#define SMC_GLOBAL
+#endif
+
 // Everything the run kernel needs to know about one model; passed by value.
 struct DevModel {
-    const DevLoc *locs;
-    const DevInv *invs;
-    const DevEdge *edges;
-    const DevGuard *guards;
-    const DevUpd *upds;
-    const double *consts;
-    const DevInsn *code;
-    const double *var_init;
-    const int32_t *comp_init_loc;
-    const uint16_t *clocks;     // ids of clock variables
-    const int32_t *comp_first_loc;  // [n_comp + 1] location range of each component
+    const SMC_GLOBAL DevLoc *locs;
+    const SMC_GLOBAL DevInv *invs;
+    const SMC_GLOBAL DevEdge *edges;
+    const SMC_GLOBAL DevGuard *guards;
+    const SMC_GLOBAL DevUpd *upds;
+    const SMC_GLOBAL double *consts;
+    const SMC_GLOBAL DevInsn *code;
+    const SMC_GLOBAL double *var_init;
+    const SMC_GLOBAL int32_t *comp_init_loc;
+    const SMC_GLOBAL uint16_t *clocks;     // ids of clock variables
+    const SMC_GLOBAL int32_t *comp_first_loc;  // [n_comp + 1] location range of each component
     int32_t n_comp;
     int32_t n_vars;
     int32_t n_clocks;

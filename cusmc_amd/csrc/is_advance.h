@@ -186,14 +186,14 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                     leaf[j] = 0.0; leaft[j] = 0.0; Mb[j] = 0.0;
                     const int c = sl * C + j;
                     if (c < n) {
-                        const DevLoc L = m.locs[loc[j]];
+                        const DevLoc L = load_rec(m.locs + loc[j]);
                         if (L.klass == SMC_K_ZERO) {
                             if (zero_j == C) zero_j = j;
                         } else {
                             bool bounded = false;
                             double M = DBL_MAX;
                             for (int i = 0; i < L.n_inv; ++i) {
-                                const DevInv iv = m.invs[L.inv_first + i];
+                                const DevInv iv = load_rec(m.invs + L.inv_first + i);
                                 const double bound = iv.bound_const ? iv.bound_value : eval_ref(iv.bound, vars, m);
                                 double ttb = bound - vars[iv.var];
                                 if (iv.strict) ttb -= 1e-6;
@@ -221,7 +221,7 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                     for (uint32_t b = deadmask; b; b &= b - 1) {
                         const int j = __ffs(b) - 1;
                         const int lj = slot_get<C>(loc, j);
-                        const DevLoc L = m.locs[lj];
+                        const DevLoc L = load_rec(m.locs + lj);
                         const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
                         const double lf = r > 0.0 ? r : 0.0;
                         double lt = lf;
