@@ -68,11 +68,11 @@ def bench_plain(args) -> dict:
     sim = cs.GpuSimulator(net, device=0)
     if args.lanes:
         # a fixed launch shape (profiles pin one kernel build this way)
-        shape = {"lanes_per_run": args.lanes, "occupancy": args.occupancy}
-        policy = args.occupancy
+        shape = {"lanes_per_run": args.lanes, "occupancy": args.occupancy, "tables": args.tables}
+        policy, tables = args.occupancy, args.tables
     else:
         tuned = sim.autotune(q, seed=SEED + 999, sample_runs=max(1024, min(runs // 4, 16384)))
-        shape, policy = {}, tuned.occupancy
+        shape, policy, tables = {}, tuned.occupancy, tuned.tables
     for w in range(args.warmup):
         sim.run(q, seed=SEED + 1000 + w, n_runs=runs, **shape)
         _progress(f"warmup {w + 1}/{args.warmup}")
@@ -102,7 +102,7 @@ def bench_plain(args) -> dict:
         "dtype": "f64",
         "data": "synthetic (seeded network generated as UPPAAL XML)",
         "config": {"workload": name, "runs_per_step": runs, "lanes_per_run": results[-1].lanes_per_run,
-                   "waves": results[-1].waves, "occupancy_policy": policy,
+                   "waves": results[-1].waves, "occupancy_policy": policy, "tables_in_lds": bool(tables),
                    "launch": "fixed" if args.lanes else "autotuned"},
         "transitions_per_sec": trans / wall,
         "race_steps_per_sec": steps / wall,
@@ -311,6 +311,8 @@ def main() -> int:
     ap.add_argument("--lanes", type=int, default=0, help="fixed lanes per run (default: autotuned)")
     ap.add_argument("--occupancy", type=int, default=0, choices=[0, 2],
                     help="with --lanes: register policy (0 unrestricted, 2 two waves per SIMD)")
+    ap.add_argument("--tables", type=int, default=0, choices=[0, 1],
+                    help="with --lanes: model tables in global memory (0) or staged in LDS (1)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -58,7 +58,8 @@ class Query(C.Structure):
 
 
 class Launch(C.Structure):
-    _fields_ = [("lanes_per_run", i32), ("waves", i32), ("waves_per_cu", i32), ("occupancy", i32)]
+    _fields_ = [("lanes_per_run", i32), ("waves", i32), ("waves_per_cu", i32), ("occupancy", i32),
+                ("tables", i32)]
 
 
 class RunOutputs(C.Structure):

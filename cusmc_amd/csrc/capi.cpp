@@ -32,16 +32,22 @@ namespace {
 struct Plan {
     int S = 1, C = 1, R = 64, stride = 1;
     int occ = 0;  // run-kernel build: 0 registers unrestricted, 2 two waves per SIMD
+    bool tab = false;     // tables staged in LDS
+    int group_waves = 1;  // waves per workgroup
+    uint32_t table_bytes = 0, wave_lds = 0;
     size_t lds = 0;
     int waves = 1;
     uint32_t chunk_runs = 64;
     uint64_t total_chunks = 0;
 };
 
-Plan plan_launch(int P, int max_vars, int lanes, int occ, int waves_req, int waves_per_cu, uint64_t total_runs,
-                 uint64_t max_runs_per_inst, bool trace, int cu_count) {
+constexpr size_t kLdsPerCu = 160 * 1024;
+
+Plan plan_launch(int P, int max_vars, int lanes, int occ, int tables, uint32_t table_bytes, int waves_req,
+                 int waves_per_cu, uint64_t total_runs, uint64_t max_runs_per_inst, bool trace, int cu_count) {
     Plan pl;
     if (occ != 0 && occ != 2) throw std::runtime_error("occupancy must be 0 or 2");
+    if (tables != 0 && tables != 1) throw std::runtime_error("tables must be 0 or 1");
     pl.occ = occ;
     int S = lanes ? lanes : (P < 64 ? P : 64);
     if (S <= 0 || S > 64 || (S & (S - 1))) throw std::runtime_error("lanes_per_run must be a power of two <= 64");
@@ -51,17 +57,35 @@ Plan plan_launch(int P, int max_vars, int lanes, int occ, int waves_req, int wav
     if (pl.C > 16) throw std::runtime_error("limit: more than 16 components per lane; use more lanes per run");
     pl.R = 64 / S;
     pl.stride = (max_vars > 0 ? max_vars : 1) | 1;
-    // variables of the R packed runs, then 12 doubles of race-tree levels per lane
-    pl.lds = (size_t)pl.R * pl.stride * sizeof(double) + 64 * 12 * sizeof(double);
-    if (pl.lds > 60000) throw std::runtime_error("limit: variables of the packed runs exceed the LDS budget");
+    // per wave: variables of the R packed runs, then two doubles per lane
+    // for each butterfly level of the race tree
+    int levels = 0;
+    while ((1 << levels) < S) ++levels;
+    const size_t wave = ((size_t)pl.R * pl.stride * sizeof(double) + (size_t)2 * levels * 64 * sizeof(double) + 15) & ~(size_t)15;
+    if (wave > 60000) throw std::runtime_error("limit: variables of the packed runs exceed the LDS budget");
+    pl.wave_lds = (uint32_t)wave;
+    pl.tab = tables == 1 && !trace;  // digest runs (parity checks) read the tables from global memory
+    if (pl.tab) {
+        if (table_bytes == 0 || table_bytes % 16) throw std::runtime_error("internal: table size");
+        pl.table_bytes = table_bytes;
+        pl.group_waves = SMC_MAX_GROUP_WAVES;
+        if (table_bytes + (size_t)pl.group_waves * wave > kLdsPerCu)
+            throw std::runtime_error("limit: the model tables do not fit in LDS");
+    }
+    pl.lds = pl.table_bytes + (size_t)pl.group_waves * wave;
     int waves = waves_req;
     if (waves <= 0) {
-        int per_cu = 0;
-        hip_check(smc::sim_occupancy(pl.C, trace, pl.occ, pl.lds, &per_cu), "occupancy query");
+        int blocks = 0;
+        hip_check(smc::sim_occupancy(pl.C, trace, pl.occ, pl.tab, pl.group_waves, pl.lds, &blocks), "occupancy query");
+        int per_cu = blocks * pl.group_waves;
         if (waves_per_cu > 0 && waves_per_cu < per_cu) per_cu = waves_per_cu;
-        if (per_cu <= 0) per_cu = 1;
+        if (per_cu <= 0) {
+            if (pl.tab) throw std::runtime_error("limit: the model tables do not fit in LDS");
+            per_cu = 1;
+        }
         waves = cu_count * per_cu;
     }
+    waves = (waves + pl.group_waves - 1) / pl.group_waves * pl.group_waves;
     // chunks: enough per wave to balance uneven run lengths, at most 64 rounds of the wave's segments
     uint64_t per_wave = (total_runs + (uint64_t)waves - 1) / (uint64_t)waves;
     uint64_t k = (per_wave + 4ull * pl.R - 1) / (4ull * pl.R);
@@ -225,6 +249,7 @@ int smc_model_create(const smc_network *net, int device, smc_model **out) {
         hip_check(hipMalloc(&m->arena, total), "hipMalloc model arena");
         hip_check(hipMemcpy(m->arena, host.data(), total, hipMemcpyHostToDevice), "upload model");
         auto at = [&](int i) { return (const void *)((const uint8_t *)m->arena + parts[i].off); };
+        m->table_bytes = (uint32_t)parts[7].off;  // locs .. code: what the run kernel reads every step
         DevModel &d = m->dm;
         d.locs = (decltype(d.locs))at(0);
         d.invs = (decltype(d.invs))at(1);
@@ -268,16 +293,18 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
         if (run_begin > (~0ull) - n_runs) throw std::runtime_error("run range overflows");
         const bool trace = outputs && outputs->digest;
         const smc_launch *use =
-            (launch && (launch->lanes_per_run || launch->waves || launch->waves_per_cu || launch->occupancy))
+            (launch && (launch->lanes_per_run || launch->waves || launch->waves_per_cu || launch->occupancy ||
+                        launch->tables))
                 ? launch : (m->has_tuned ? &m->tuned : nullptr);
         hip_check(hipSetDevice(m->device), "hipSetDevice");
         Plan pl = plan_launch(m->img.n_leaves, m->img.n_vars, use ? use->lanes_per_run : 0, use ? use->occupancy : 0,
-                              use ? use->waves : 0,
+                              use ? use->tables : 0, m->table_bytes, use ? use->waves : 0,
                               use ? use->waves_per_cu : 0, n_runs, n_runs, trace, m->cu_count);
         m->runs.ensure(n_runs);
         pl.total_chunks = (n_runs + pl.chunk_runs - 1) / pl.chunk_runs;
         int waves = pl.waves;
-        if ((uint64_t)waves > pl.total_chunks) waves = (int)pl.total_chunks;
+        if ((uint64_t)waves > pl.total_chunks)
+            waves = (int)((pl.total_chunks + pl.group_waves - 1) / pl.group_waves * pl.group_waves);
 
         m->h_inst.m = m->dm;
         m->h_inst.p = make_params(q, m->img, seed, run_begin, n_runs, m->runs, outputs && outputs->time, trace);
@@ -291,12 +318,15 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
         lp.total_chunks = pl.total_chunks;
         lp.counter = m->d_counter;
         lp.round = nullptr;
+        lp.tables = pl.tab ? (decltype(lp.tables))m->arena : nullptr;
+        lp.table_bytes = pl.table_bytes;
+        lp.wave_lds = pl.wave_lds;
         const uint32_t blocks = reduce_blocks(n_runs);
 
         hip_check(hipMemcpyAsync(m->d_inst, &m->h_inst, sizeof(InstDesc), hipMemcpyHostToDevice, m->stream), "upload instance");
         hip_check(smc::launch_prep(m->d_counter, nullptr, 0, m->stream), "prep kernel launch");
         hip_check(hipEventRecord(m->ev[0], m->stream), "event");
-        hip_check(smc::launch_sim(lp, pl.C, trace, pl.occ, waves, pl.lds, m->stream), "run kernel launch");
+        hip_check(smc::launch_sim(lp, pl.C, trace, pl.occ, waves, pl.group_waves, pl.lds, m->stream), "run kernel launch");
         hip_check(hipEventRecord(m->ev[1], m->stream), "event");
         hip_check(smc::launch_reduce(m->d_inst, 1, blocks, m->d_partials, m->d_acc, 0, m->stream), "reduce kernel launch");
         hip_check(hipEventRecord(m->ev[2], m->stream), "event");
@@ -332,7 +362,8 @@ int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sampl
         std::vector<smc_launch> cands;
         for (int S = P < 64 ? P : 64; S >= 1; S >>= 1) {
             if (P / S > 16) break;
-            for (int occ : {0, 2}) cands.push_back(smc_launch{S, 0, 0, occ});
+            for (int occ : {0, 2})
+                for (int tab : {0, 1}) cands.push_back(smc_launch{S, 0, 0, occ, tab});
         }
         // each candidate runs at least twice the runs its grid holds at once,
         // so the sample measures the steady state rather than the tail
@@ -341,8 +372,8 @@ int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sampl
         for (const smc_launch &c : cands) {
             uint64_t n = sample_runs;
             try {
-                Plan pl = plan_launch(P, m->img.n_vars, c.lanes_per_run, c.occupancy, 0, 0, sample_runs, sample_runs,
-                                      false, m->cu_count);
+                Plan pl = plan_launch(P, m->img.n_vars, c.lanes_per_run, c.occupancy, c.tables, m->table_bytes, 0, 0,
+                                      sample_runs, sample_runs, false, m->cu_count);
                 n = std::max<uint64_t>(n, 2ull * (uint64_t)pl.waves * (uint64_t)pl.R);
             } catch (const std::exception &) {
                 continue;  // e.g. LDS budget exceeded at this packing
@@ -463,8 +494,8 @@ int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_bat
                 maxr = std::max<uint64_t>(maxr, items[i].n_runs);
                 max_vars = std::max(max_vars, items[i].model->img.n_vars);
             }
-            g.pl = plan_launch(std::get<0>(kv.first), max_vars, std::get<1>(kv.first), std::get<2>(kv.first), 0, 0, total,
-                               maxr, false, cu);
+            g.pl = plan_launch(std::get<0>(kv.first), max_vars, std::get<1>(kv.first), std::get<2>(kv.first), 0, 0, 0,
+                               0, total, maxr, false, cu);
             std::vector<InstDesc> inst;
             uint64_t chunk = 0;
             for (int i : g.members) {
@@ -504,6 +535,9 @@ int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_bat
             g.lp.total_chunks = g.pl.total_chunks;
             g.lp.counter = g.d_counter;
             g.lp.round = b->d_round;
+            g.lp.tables = nullptr;
+            g.lp.table_bytes = 0;
+            g.lp.wave_lds = g.pl.wave_lds;
         }
         b->gstreams.resize(b->groups.size());
         b->joins.resize(b->groups.size());
@@ -520,7 +554,7 @@ int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_bat
         for (size_t k = 0; k < b->groups.size(); ++k) {
             auto &g = b->groups[k];
             hip_check(hipStreamWaitEvent(b->gstreams[k], b->fork, 0), "fork wait");
-            hip_check(smc::launch_sim(g.lp, g.pl.C, false, g.pl.occ, g.pl.waves, g.pl.lds, b->gstreams[k]),
+            hip_check(smc::launch_sim(g.lp, g.pl.C, false, g.pl.occ, g.pl.waves, g.pl.group_waves, g.pl.lds, b->gstreams[k]),
                       "capture run kernel");
             hip_check(hipEventRecord(b->joins[k], b->gstreams[k]), "join");
             hip_check(hipStreamWaitEvent(b->stream, b->joins[k], 0), "join wait");

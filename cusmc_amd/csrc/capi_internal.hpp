@@ -82,6 +82,7 @@ struct smc_model {
     smc::ModelImage img;
     DevModel dm{};
     void *arena = nullptr;
+    uint32_t table_bytes = 0;  // arena prefix a run kernel may stage in LDS (the hot tables)
     int cu_count = 0;
     smc::capi::RunBuffers runs;
     InstDesc h_inst{};

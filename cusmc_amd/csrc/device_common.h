@@ -102,13 +102,42 @@ __device__ __forceinline__ T load_rec(const SMC_GLOBAL T *p) {
     return r;
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// The same from LDS (records 16-byte aligned there too).
+template <class T>
+__device__ __forceinline__ T load_rec(const SMC_LDS T *p) {
+    static_assert(sizeof(T) % 8 == 0, "record size");
+    T r;
+    if constexpr (sizeof(T) % 16 == 0) {
+        const SMC_LDS uint4 *q = (const SMC_LDS uint4 *)p;
+        uint4 v[sizeof(T) / 16];
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) / 16); ++i) {
+            v[i] = q[i];
+            asm("" : "+v"(v[i].x), "+v"(v[i].y), "+v"(v[i].z), "+v"(v[i].w));
+        }
+        __builtin_memcpy(&r, v, sizeof(T));
+    } else {
+        const SMC_LDS uint2 *q = (const SMC_LDS uint2 *)p;
+        uint2 v[sizeof(T) / 8];
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) / 8); ++i) {
+            v[i] = q[i];
+            asm("" : "+v"(v[i].x), "+v"(v[i].y));
+        }
+        __builtin_memcpy(&r, v, sizeof(T));
+    }
+    return r;
+}
+#endif
+
 // ------------------------------------------------------------ expressions
 
 // Register-bytecode interpreter (expressions.cu:10-119).  Inlined: an
 // out-of-line call costs the whole kernel registers and time even on models
 // that never reach it.
-__device__ __forceinline__ double eval_code(uint32_t ref, const double *vars, const SMC_GLOBAL double *consts,
-                                            const SMC_GLOBAL DevInsn *code) {
+template <class PC, class PI>
+__device__ __forceinline__ double eval_code(uint32_t ref, const double *vars, PC consts, PI code) {
     const uint32_t first = ref & 0xFFFFFFu, count = (ref >> 24) & 0x3Fu;
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
     for (uint32_t k = 0; k < count; ++k) {
@@ -131,7 +160,8 @@ __device__ __forceinline__ double eval_code(uint32_t ref, const double *vars, co
 }
 
 // Value of an expression reference: constants and variables inline.
-__device__ __forceinline__ double eval_ref(uint32_t ref, const double *vars, const DevModel &m) {
+template <class M>
+__device__ __forceinline__ double eval_ref(uint32_t ref, const double *vars, const M &m) {
     const uint32_t kind = ref >> 30;
     if (kind == SMC_REF_CONST) return m.consts[ref & 0x3FFFFFFFu];
     if (kind == SMC_REF_VAR) return vars[ref & 0x3FFFFFFFu];
@@ -154,8 +184,8 @@ __device__ __forceinline__ bool guard_holds(int op, double x, double r) {
 // Enabled edges of location `loc` as a bit mask over its edge list:
 // spontaneous edges (recv_chan == 0) or receive edges on channel recv_chan
 // (check_enabled_edges simulation.cu:268-305, SyncInSerial :363-377).
-__device__ __forceinline__ uint64_t enabled_mask(int loc, int recv_chan, const double *vars,
-                                                 const DevModel &m) {
+template <class M>
+__device__ __forceinline__ uint64_t enabled_mask(int loc, int recv_chan, const double *vars, const M &m) {
     const DevLoc L = load_rec(m.locs + loc);
     const int ne = L.edge_n;
     uint64_t mask = 0;
@@ -173,7 +203,8 @@ __device__ __forceinline__ uint64_t enabled_mask(int loc, int recv_chan, const d
     return mask;
 }
 
-__device__ __forceinline__ long long edge_weight(int e, const double *vars, const DevModel &m) {
+template <class M>
+__device__ __forceinline__ long long edge_weight(int e, const double *vars, const M &m) {
     const int w = smc_to_int(eval_ref(m.edges[e].weight, vars, m));
     return w > 0 ? w : 0;
 }
@@ -181,9 +212,10 @@ __device__ __forceinline__ long long edge_weight(int e, const double *vars, cons
 // Weighted choice among enabled edges (take_transition simulation.cu:43-79
 // and get_sum_edge_weight :9-24, with the weights of the enabled edges).
 // The draw is made only when there is a choice.
+template <class M>
 __device__ __forceinline__ int pick_edge(uint64_t mask, int edge_first, uint64_t seed, uint64_t run,
                                          uint32_t step, uint32_t purpose, uint32_t comp,
-                                         const double *vars, const DevModel &m) {
+                                         const double *vars, const M &m) {
     if ((mask & (mask - 1)) == 0) return lowest_lane(mask);
     const smc_u32x4 w = smc_draw(seed, run, step, purpose, comp);
     const double u = smc_u01(w.v[0], w.v[1]);
@@ -206,9 +238,9 @@ __device__ __forceinline__ int pick_edge(uint64_t mask, int edge_first, uint64_t
 // racer is exponential; the transition that crosses the time bound still
 // competes with their first null event after the bound, which is drawn from
 // this sum (memoryless races).
-template <int C>
+template <int C, class M>
 __device__ __forceinline__ double dead_total(uint32_t deadmask, const int (&loc)[C], const double *vars,
-                                             const DevModel &m, int S) {
+                                             const M &m, int S) {
     double dl[C];
 #pragma unroll
     for (int j = 0; j < C; ++j) {
@@ -229,7 +261,8 @@ struct LaneQuery {
 
 // Fire edge `e`: updates in order, query variable tracking, new location
 // (take_transition simulation.cu:88-180).  Returns the destination.
-__device__ __forceinline__ int fire_edge(int e, double *vars, const DevModel &m, LaneQuery &q,
+template <class M>
+__device__ __forceinline__ int fire_edge(int e, double *vars, const M &m, LaneQuery &q,
                                          bool &goal, int &chan) {
     const DevEdge E = load_rec(m.edges + e);
     for (int k = 0; k < E.upd_n; ++k) {
@@ -252,8 +285,8 @@ __device__ __forceinline__ int fire_edge(int e, double *vars, const DevModel &m,
 // exponential location).  Zero-delay and clock-bounded locations with an
 // enabled edge can fire.  Nothing but time changes such a state, so the run's
 // outcome is decided; the kernels check it after every change of state.
-template <int C>
-__device__ __forceinline__ bool absorbed(const int (&loc)[C], const double *vars, const DevModel &m, int sl, int n,
+template <int C, class M>
+__device__ __forceinline__ bool absorbed(const int (&loc)[C], const double *vars, const M &m, int sl, int n,
                                          uint64_t seg_mask) {
     bool can = false;
 #pragma unroll
@@ -286,9 +319,9 @@ __device__ __forceinline__ void slot_set(T (&a)[C], int j, T v);
 // then slot order is component order).  TRACE mixes the fired edges into
 // the digest in component order.  Returns false, having changed nothing,
 // when some enabled edge does not commute.
-template <int C>
+template <int C, class M>
 __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int (&loc)[C], double *vars,
-                                                    const DevModel &m, LaneQuery &q, bool &goal, uint64_t seed,
+                                                    const M &m, LaneQuery &q, bool &goal, uint64_t seed,
                                                     uint64_t run, uint32_t step, int sl, int S, int seg_base,
                                                     uint64_t seg_mask, uint32_t &trans, bool trace,
                                                     uint64_t &digest) {
@@ -388,9 +421,9 @@ __device__ __forceinline__ void rotate_slots(T (&a)[C]) {
 // (SyncInSerial simulation.cu:378-404): the lanes holding enabled receivers
 // in lane order, their slots in slot order.  Enabled sets mk were computed
 // before any of them fires; each picks with its own stream.
-template <int C>
+template <int C, class M>
 __device__ __forceinline__ void fire_receivers_serial(uint64_t (&mk)[C], int (&loc)[C], bool any, double *vars,
-                                                      const DevModel &m, LaneQuery &q, bool &goal, uint64_t seed,
+                                                      const M &m, LaneQuery &q, bool &goal, uint64_t seed,
                                                       uint64_t run, uint32_t step, int lane, int seg_base,
                                                       uint64_t seg_mask, uint32_t &trans, bool trace,
                                                       uint64_t &digest) {

@@ -118,22 +118,34 @@ struct DevUpd {             // 16 bytes
 // The tables live in device global memory.  Device code sees their pointers
 // in the global address space, so reads compile to global loads (their own
 // wait counter) rather than flat loads, which the compiler must also order
-// against LDS traffic; host code sees plain pointers of the same size.
+// against LDS traffic; host code sees plain pointers of the same size.  A
+// run kernel may stage the tables in LDS and read them through a second
+// instance of the same record (DevModelT<SmcLdsTables>).
 #if defined(__HIP_DEVICE_COMPILE__)
 #define SMC_GLOBAL __attribute__((address_space(1)))
+#define SMC_LDS __attribute__((address_space(3)))
 #else
 #define SMC_GLOBAL
+#define SMC_LDS
 #endif
 
+struct SmcGlobalTables {
+    template <class T> using ptr = const SMC_GLOBAL T *;
+};
+struct SmcLdsTables {
+    template <class T> using ptr = const SMC_LDS T *;
+};
+
 // Everything the run kernel needs to know about one model; passed by value.
-struct DevModel {
-    const SMC_GLOBAL DevLoc *locs;
-    const SMC_GLOBAL DevInv *invs;
-    const SMC_GLOBAL DevEdge *edges;
-    const SMC_GLOBAL DevGuard *guards;
-    const SMC_GLOBAL DevUpd *upds;
-    const SMC_GLOBAL double *consts;
-    const SMC_GLOBAL DevInsn *code;
+template <class A>
+struct DevModelT {
+    typename A::template ptr<DevLoc> locs;
+    typename A::template ptr<DevInv> invs;
+    typename A::template ptr<DevEdge> edges;
+    typename A::template ptr<DevGuard> guards;
+    typename A::template ptr<DevUpd> upds;
+    typename A::template ptr<double> consts;
+    typename A::template ptr<DevInsn> code;
     const SMC_GLOBAL double *var_init;
     const SMC_GLOBAL int32_t *comp_init_loc;
     const SMC_GLOBAL uint16_t *clocks;     // ids of clock variables
@@ -145,3 +157,4 @@ struct DevModel {
     int32_t n_locs;
     int32_t n_edges;
 };
+using DevModel = DevModelT<SmcGlobalTables>;

@@ -42,7 +42,13 @@ struct LaunchParams {
     uint64_t total_chunks;
     unsigned long long *counter;  // chunks handed out (cleared by launch_prep)
     const uint64_t *round;      // optional: runs of round r start at run_begin + r * n_runs
+    const SMC_GLOBAL uint8_t *tables;  // tables staged in LDS (one instance): arena start
+    uint32_t table_bytes;       // bytes staged (multiple of 16), 0 without staging
+    uint32_t wave_lds;          // LDS bytes of one wave after the tables (multiple of 16)
 };
+
+// Waves per workgroup of the run kernel, at most.
+#define SMC_MAX_GROUP_WAVES 4
 
 struct ReducePartial {
     double sum, sumsq, vmin, vmax;
@@ -54,8 +60,9 @@ namespace smc {
 
 // One build of the run kernel (sim_variant.hip).
 struct SimVariant {
-    hipError_t (*launch)(const LaunchParams &lp, bool trace, int waves, size_t lds, hipStream_t s);
-    hipError_t (*occupancy)(bool trace, size_t lds, int *blocks_per_cu);
+    hipError_t (*launch)(const LaunchParams &lp, bool trace, bool tab, int waves, int group_waves, size_t lds,
+                         hipStream_t s);
+    hipError_t (*occupancy)(bool trace, bool tab, int group_waves, size_t lds, int *blocks_per_cu);
 };
 SimVariant sim_variant_c1_o0();
 SimVariant sim_variant_c2_o0();
@@ -68,10 +75,13 @@ SimVariant sim_variant_c4_o2();
 SimVariant sim_variant_c8_o2();
 SimVariant sim_variant_c16_o2();
 
-// occ: 0 registers unrestricted, 2 at most 256 (two waves per SIMD)
-hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int occ, int waves, size_t lds_bytes,
-                      hipStream_t stream);
-hipError_t sim_occupancy(int slots, bool trace, int occ, size_t lds_bytes, int *blocks_per_cu);
+// occ: 0 registers unrestricted, 2 at most 256 (two waves per SIMD); tab:
+// tables staged in LDS (lp.table_bytes > 0); waves: total, a multiple of
+// group_waves (waves per workgroup, <= SMC_MAX_GROUP_WAVES)
+hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int occ, int waves, int group_waves,
+                      size_t lds_bytes, hipStream_t stream);
+hipError_t sim_occupancy(int slots, bool trace, int occ, bool tab, int group_waves, size_t lds_bytes,
+                         int *blocks_per_cu);
 hipError_t launch_reduce(const InstDesc *inst, int n_inst, uint32_t blocks_per_inst, ReducePartial *partials,
                          ReducePartial *acc, int accumulate, hipStream_t s);
 hipError_t launch_prep(unsigned long long *counter, uint64_t *round, int advance, hipStream_t s);

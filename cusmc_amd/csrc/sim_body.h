@@ -39,6 +39,7 @@
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <stdint.h>
+#include <type_traits>
 
 #include "device_common.h"
 #include "kernels.hpp"
@@ -48,26 +49,53 @@ namespace {
 // ------------------------------------------------------------ run kernel
 
 
+// Table pointer of the global arena rebased onto its LDS copy.
+template <class T>
+__device__ __forceinline__ const SMC_LDS T *in_lds(const SMC_GLOBAL T *p, const SMC_GLOBAL uint8_t *base,
+                                                   const SMC_LDS uint8_t *copy) {
+    return (const SMC_LDS T *)(copy + ((const SMC_GLOBAL uint8_t *)p - base));
+}
+
 // One wave at a time takes a chunk of runs of one instance from the launch's
 // queue, so the instance (its tables, query and outputs) is uniform across
 // the wave and lives in scalar registers; the wave's segments then pull the
-// chunk's runs one by one from a counter in LDS.
-template <int C, bool TRACE>
+// chunk's runs one by one from a counter in LDS.  A workgroup holds
+// blockDim/64 independent waves.  With TAB (single-instance launches) the
+// workgroup first copies the model's tables into LDS and every wave reads
+// them there: the race and the transitions are chains of dependent table
+// reads, and LDS answers them several times sooner than the caches.
+//
+// LDS: [tables: lp.table_bytes, TAB only] then per wave [variables of the
+// wave's 64/S runs: var_stride doubles each][race-tree levels: 2 log2(S)
+// doubles per lane, level-major].
+template <int C, bool TRACE, bool TAB>
 __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
-    extern __shared__ double lds[];
-    __shared__ unsigned int chunk_next;
-    const int lane = threadIdx.x;
+    extern __shared__ __attribute__((aligned(16))) double lds[];
+    __shared__ unsigned int chunk_next_of[SMC_MAX_GROUP_WAVES];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    unsigned int &chunk_next = chunk_next_of[wv];
     const int S = lp.lanes;
     const int sl = lane & (S - 1);
     const int seg_base = lane - sl;
     const uint64_t seg_mask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << seg_base);
-    double *vars = lds + (size_t)(lane / S) * lp.var_stride;
-    // per lane: the race tree's partial sums at each butterfly level
-    // (own[k] at [k], sibling block's at [6 + k]), kept in LDS rather than
-    // registers from the sum to the descent
-    double *lvl = lds + (size_t)(64 / S) * lp.var_stride + (size_t)lane * 12;
+    double *wbase = lds + (TAB ? lp.table_bytes / 8 : 0) + (size_t)wv * (lp.wave_lds / 8);
+    double *vars = wbase + (size_t)(lane / S) * lp.var_stride;
+    // per lane: the race tree's partial sums at each butterfly level (own at
+    // [64 (2k)], sibling block's at [64 (2k + 1)]), kept in LDS rather than
+    // registers from the sum to the descent; level-major, so the 64 lanes
+    // touch 64 consecutive doubles (no bank conflicts)
+    double *lvl = wbase + (size_t)(64 / S) * lp.var_stride + lane;
     const double INF = __builtin_huge_val();
     const uint64_t round = lp.round ? *lp.round : 0;
+    using Tables = typename std::conditional<TAB, SmcLdsTables, SmcGlobalTables>::type;
+    const SMC_LDS uint8_t *tab = (const SMC_LDS uint8_t *)(SMC_LDS double *)lds;
+    if (TAB) {
+        const SMC_GLOBAL uint4 *src = (const SMC_GLOBAL uint4 *)lp.tables;
+        SMC_LDS uint4 *dst = (SMC_LDS uint4 *)(SMC_LDS double *)lds;
+        for (uint32_t i = threadIdx.x; i < lp.table_bytes / 16; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
 
   for (;;) {
     unsigned long long g = 0;
@@ -80,7 +108,29 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
         if (lp.inst[mid].chunk_first <= g) lo = mid;
         else hi = mid - 1;
     }
-    const DevModel m = lp.inst[lo].m;
+    const DevModel mg = lp.inst[lo].m;
+    DevModelT<Tables> m;
+    if constexpr (TAB) {
+        m.locs = in_lds(mg.locs, lp.tables, tab);
+        m.invs = in_lds(mg.invs, lp.tables, tab);
+        m.edges = in_lds(mg.edges, lp.tables, tab);
+        m.guards = in_lds(mg.guards, lp.tables, tab);
+        m.upds = in_lds(mg.upds, lp.tables, tab);
+        m.consts = in_lds(mg.consts, lp.tables, tab);
+        m.code = in_lds(mg.code, lp.tables, tab);
+        m.var_init = mg.var_init;
+        m.comp_init_loc = mg.comp_init_loc;
+        m.clocks = mg.clocks;
+        m.comp_first_loc = mg.comp_first_loc;
+        m.n_comp = mg.n_comp;
+        m.n_vars = mg.n_vars;
+        m.n_clocks = mg.n_clocks;
+        m.n_leaves = mg.n_leaves;
+        m.n_locs = mg.n_locs;
+        m.n_edges = mg.n_edges;
+    } else {
+        m = mg;
+    }
     const SimParams p = lp.inst[lo].p;
     const uint64_t chunk = g - lp.inst[lo].chunk_first;
     const uint32_t r_first = (uint32_t)(chunk * lp.chunk_runs);
@@ -228,8 +278,8 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                         for (int k = 0; k < 6; ++k) {
                             if ((1 << k) < S) {
                                 const double sib = seg_xchg(s, k);
-                                lvl[k] = s;
-                                lvl[6 + k] = sib;
+                                lvl[64 * (2 * k)] = s;
+                                lvl[64 * (2 * k + 1)] = sib;
                                 s = s + sib;
                             }
                         }
@@ -245,7 +295,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                             for (int k = 5; k >= 0; --k) {
                                 if ((1 << k) < S) {
                                     const bool in_left = ((sl >> k) & 1) == 0;
-                                    const double own = lvl[k], sib = lvl[6 + k];
+                                    const double own = lvl[64 * (2 * k)], sib = lvl[64 * (2 * k + 1)];
                                     const double left = in_left ? own : sib;
                                     const double right = in_left ? sib : own;
                                     if (!(v < left || right == 0.0)) { v = v - left; b += 1 << k; }
@@ -385,18 +435,19 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
   }
 }
 
-// Two builds of the run kernel: registers unrestricted (one wave per SIMD
-// once the kernel needs more than 256), and at most 256 (two waves per SIMD,
-// spilling what does not fit).  Which is faster depends on the packing; the
-// autotuner measures both.
-template <int C, bool TRACE>
-__global__ __launch_bounds__(64) void sim_kernel(LaunchParams lp) {
-    sim_body<C, TRACE>(lp);
+// Two register policies of the run kernel: unrestricted (one wave per SIMD
+// once the kernel needs more than 256 registers), and at most 256 (two waves
+// per SIMD, spilling what does not fit).  Which is faster depends on the
+// packing; the autotuner measures both.
+template <int C, bool TRACE, bool TAB>
+__global__ __launch_bounds__(64 * SMC_MAX_GROUP_WAVES) void sim_kernel(LaunchParams lp) {
+    sim_body<C, TRACE, TAB>(lp);
 }
 
-template <int C, bool TRACE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void sim_kernel_o2(LaunchParams lp) {
-    sim_body<C, TRACE>(lp);
+template <int C, bool TRACE, bool TAB>
+__global__ __launch_bounds__(64 * SMC_MAX_GROUP_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void sim_kernel_o2(
+    LaunchParams lp) {
+    sim_body<C, TRACE, TAB>(lp);
 }
 
 }  // namespace

@@ -110,17 +110,18 @@ static SimVariant variant(int slots, int occ) {
     }
 }
 
-hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int occ, int waves, size_t lds_bytes,
-                      hipStream_t stream) {
+hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int occ, int waves, int group_waves,
+                      size_t lds_bytes, hipStream_t stream) {
     const SimVariant v = variant(slots, occ);
-    if (!v.launch) return hipErrorInvalidValue;
-    return v.launch(lp, trace, waves, lds_bytes, stream);
+    if (!v.launch || group_waves < 1 || group_waves > SMC_MAX_GROUP_WAVES || waves % group_waves) return hipErrorInvalidValue;
+    return v.launch(lp, trace, lp.table_bytes > 0, waves, group_waves, lds_bytes, stream);
 }
 
-hipError_t sim_occupancy(int slots, bool trace, int occ, size_t lds_bytes, int *blocks_per_cu) {
+hipError_t sim_occupancy(int slots, bool trace, int occ, bool tab, int group_waves, size_t lds_bytes,
+                         int *blocks_per_cu) {
     const SimVariant v = variant(slots, occ);
     if (!v.occupancy) return hipErrorInvalidValue;
-    return v.occupancy(trace, lds_bytes, blocks_per_cu);
+    return v.occupancy(trace, tab, group_waves, lds_bytes, blocks_per_cu);
 }
 
 hipError_t launch_reduce(const InstDesc *inst, int n_inst, uint32_t blocks_per_inst, ReducePartial *partials,

@@ -1,7 +1,8 @@
 // sim_variant.hip — one build of the run kernel (sim_body.h): SMC_C
 // component slots per lane, register policy SMC_OCC (0 unrestricted, 2 at
-// most 256 registers: two waves per SIMD).  The build compiles this file
-// once per pair, in parallel.
+// most 256 registers: two waves per SIMD): with digests (tables in global
+// memory), and without, with the tables in global memory or staged in LDS.
+// The build compiles this file once per pair, in parallel.
 #include "sim_body.h"
 
 #if !defined(SMC_C) || !defined(SMC_OCC)
@@ -13,29 +14,33 @@
 
 namespace {
 
-template <bool TRACE>
-const void *kernel_fn() {
 #if SMC_OCC == 2
-    return (const void *)sim_kernel_o2<SMC_C, TRACE>;
+#define SMC_KERNEL sim_kernel_o2
 #else
-    return (const void *)sim_kernel<SMC_C, TRACE>;
+#define SMC_KERNEL sim_kernel
 #endif
+
+// Digest runs (parity checks) read the tables from global memory only.
+const void *kernel_fn(bool trace, bool tab) {
+    if (trace) return (const void *)SMC_KERNEL<SMC_C, true, false>;
+    return tab ? (const void *)SMC_KERNEL<SMC_C, false, true> : (const void *)SMC_KERNEL<SMC_C, false, false>;
 }
 
-hipError_t launch(const LaunchParams &lp, bool trace, int waves, size_t lds, hipStream_t s) {
-#if SMC_OCC == 2
-    if (trace) hipLaunchKernelGGL((sim_kernel_o2<SMC_C, true>), dim3(waves), dim3(64), lds, s, lp);
-    else hipLaunchKernelGGL((sim_kernel_o2<SMC_C, false>), dim3(waves), dim3(64), lds, s, lp);
-#else
-    if (trace) hipLaunchKernelGGL((sim_kernel<SMC_C, true>), dim3(waves), dim3(64), lds, s, lp);
-    else hipLaunchKernelGGL((sim_kernel<SMC_C, false>), dim3(waves), dim3(64), lds, s, lp);
-#endif
+hipError_t launch(const LaunchParams &lp, bool trace, bool tab, int waves, int group_waves, size_t lds,
+                  hipStream_t s) {
+    const dim3 grid(waves / group_waves), block(64 * group_waves);
+    if (trace) {
+        if (tab) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((SMC_KERNEL<SMC_C, true, false>), grid, block, lds, s, lp);
+    } else {
+        if (tab) hipLaunchKernelGGL((SMC_KERNEL<SMC_C, false, true>), grid, block, lds, s, lp);
+        else hipLaunchKernelGGL((SMC_KERNEL<SMC_C, false, false>), grid, block, lds, s, lp);
+    }
     return hipGetLastError();
 }
 
-hipError_t occupancy(bool trace, size_t lds, int *blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace ? kernel_fn<true>() : kernel_fn<false>(),
-                                                        64, lds);
+hipError_t occupancy(bool trace, bool tab, int group_waves, size_t lds, int *blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kernel_fn(trace, tab), 64 * group_waves, lds);
 }
 
 }  // namespace

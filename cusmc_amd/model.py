@@ -145,7 +145,7 @@ class GpuSimulator:
 
     def run(self, query: A.Query, *, seed: int, n_runs: int, run_begin: int = 0,
             lanes_per_run: int = 0, waves: int = 0, waves_per_cu: int = 0, occupancy: int = 0,
-            per_run: tuple[str, ...] = ()) -> RunResult:
+            tables: int = 0, per_run: tuple[str, ...] = ()) -> RunResult:
         outs = A.RunOutputs()
         arrays = {}
         spec = {"flag": np.uint8, "value": np.float64, "steps": np.uint32, "transitions": np.uint32,
@@ -156,7 +156,7 @@ class GpuSimulator:
             arr = np.zeros(n_runs, dtype=spec[name])
             arrays[name] = arr
             setattr(outs, name, arr.ctypes.data_as(C.POINTER(ctype[name])))
-        launch = A.Launch(int(lanes_per_run), int(waves), int(waves_per_cu), int(occupancy))
+        launch = A.Launch(int(lanes_per_run), int(waves), int(waves_per_cu), int(occupancy), int(tables))
         res = A.Result()
         A.check(A.lib().smc_run(self._h, C.byref(query), C.c_uint64(seed), C.c_uint64(run_begin),
                                 C.c_uint64(n_runs), C.byref(launch), C.byref(outs), C.byref(res)))

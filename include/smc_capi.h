@@ -77,6 +77,10 @@ typedef struct smc_launch {
     int32_t waves_per_cu;      /* occupancy target used when waves == 0     */
     int32_t occupancy;         /* 0: registers unrestricted; 2: at most 256
                                   registers, two waves per SIMD            */
+    int32_t tables;            /* 0: model tables read from global memory;
+                                  1: staged in LDS, one copy per workgroup
+                                  of four waves (smc_run only; batches read
+                                  them from global memory)                 */
 } smc_launch;
 
 /* Per-run outputs, all optional (NULL = not wanted), each n_runs long. */
@@ -122,7 +126,8 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed,
 /*
  * Autotuner: time a sample of `sample_runs` runs of `q` with each candidate
  * launch shape (lanes per run from next_pow2(components) down to a
- * sixteenth of it, i.e. 1-16 components per lane, and occupancy caps) and
+ * sixteenth of it, i.e. 1-16 components per lane, occupancy caps, and
+ * tables in global memory or in LDS when they fit) and
  * keep the fastest in the model; smc_run uses it whenever its launch
  * argument is NULL or all zero.  The chosen shape is written to *best.
  * Results of smc_run do not depend on the shape (bit-identical per run).

@@ -93,6 +93,38 @@ def test_bit_exact_vs_oracle(case, models_dir):
         sim.close()
 
 
+# Runs without digests may stage the model tables in LDS (workgroups of four
+# waves): the same per-run results as the oracle, for both register policies.
+TABLE_CASES = ["RaceTest", "UppaalWeightTest", "fischer_10_29", "aloha_25", "covid_32", "race8", "fischer32",
+               "sir64"]
+
+
+@pytest.mark.parametrize("name", TABLE_CASES)
+def test_lds_tables_bit_exact_vs_oracle(name, models_dir):
+    case = next(c for c in CASES if c[0] == name)
+    _, src, locq, qspec, runs, lane_opts = case
+    net = _net(models_dir, src, locq)
+    q = _query(net, qspec)
+    seed = 0x7AB0000 + len(name)
+    keys = ("flag", "value", "steps", "transitions", "time")
+    ref, ro = oracle.run(net, q, seed=seed, n_runs=runs, run_begin=3, threads=8, per_run=keys)
+    sim = cs.GpuSimulator(net)
+    try:
+        for S in lane_opts:
+            for occ in (0, 2):
+                print(f"[gpu parity, tables in LDS] {name} lanes_per_run={S} occupancy={occ} runs={runs}", flush=True)
+                res = sim.run(q, seed=seed, n_runs=runs, run_begin=3, lanes_per_run=S, occupancy=occ, tables=1,
+                              per_run=keys)
+                for k in keys:
+                    a, b = res.per_run[k], ro[k]
+                    same = np.array_equal(a.view(np.uint64), b.view(np.uint64)) if k in ("value", "time") \
+                        else np.array_equal(a, b)
+                    assert same, f"{name} S={S} occupancy={occ}: per-run {k} differs from the oracle"
+                assert res.hits == ref.hits and res.steps == ref.steps
+    finally:
+        sim.close()
+
+
 def test_run_range_split_is_consistent(models_dir):
     """Runs are addressed by global index: two calls over halves equal one call."""
     net = _net(models_dir, "fischer:32", "P0.cs")
