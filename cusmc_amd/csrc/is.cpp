@@ -19,10 +19,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "capi_internal.hpp"
@@ -140,6 +142,9 @@ struct smc_is {
     size_t lds = 0;
     int n_locs = 0, n_vars = 0, n_comp = 0, F = 0, fpad = 0, H = 128, Lpad = 0;
     int n_groups = 0, Gpad = 0;  // tilt groups: the MLP's outputs
+    int G16 = 0;                 // groups rounded up to 16: the outputs the fused forward pass computes
+    bool fused_fwd = false;      // forward pass in one kernel (weights fit in LDS)
+    DevBuf<int32_t> tix_loc, tix_group;  // per location: its theta column (per location / per group), -1 if untiltable
     size_t o_w1 = 0, o_b1 = 0, o_w2 = 0, o_b2 = 0, o_w3 = 0, o_b3 = 0, n_params = 0;
     double qinit = 0.0;
     std::vector<uint8_t> h_tiltable;
@@ -158,7 +163,7 @@ struct smc_is {
     DevBuf<uint64_t> runs_in;
     DevBuf<float> weight;
     DevBuf<uint16_t> h1, h2;
-    DevBuf<float> theta, thg;
+    DevBuf<float> thg;
     // recorded (run, macro-step) pairs
     DevBuf<uint16_t> rX;
     DevBuf<float> rA, rB, rw, rAg, rBg;  // per location, and summed per tilt group
@@ -188,9 +193,10 @@ struct smc_is {
         runs_in.ensure(r, "hipMalloc rows");
         weight.ensure(r, "hipMalloc rows");
         if (feats) {
-            h1.ensure(r * H, "hipMalloc activations");
-            h2.ensure(r * H, "hipMalloc activations");
-            theta.ensure(r * Lpad, "hipMalloc tilts");
+            if (!fused_fwd) {
+                h1.ensure(r * H, "hipMalloc activations");
+                h2.ensure(r * H, "hipMalloc activations");
+            }
             thg.ensure(r * Gpad, "hipMalloc tilts");
         }
     }
@@ -228,6 +234,7 @@ struct smc_is {
         P.is_max = q.is_max ? 1 : 0;
         P.max_steps = q.max_steps ? q.max_steps : SMC_DEFAULT_MAX_STEPS;
         P.tiltable = tiltable.p;
+        P.tilt_index = tix_loc.p;
         P.var_scale = var_scale.p;
         P.weight = weight.p;
         P.active_count = counters.p;
@@ -235,23 +242,42 @@ struct smc_is {
     }
 
     // theta = MLP(X) for `rows` rows (a multiple of 64): the group outputs in
-    // og (rows x Gpad), expanded to every location in out (rows x Lpad);
-    // a1, a2 keep the bf16 hidden activations for the backward pass.
-    void forward(const uint16_t *Xin, int rows, uint16_t *a1, uint16_t *a2, float *og, float *out) {
-        GemmArgs g1{};
-        g1.A = Xin; g1.lda = fpad; g1.B = wb.p + o_w1; g1.ldb = H; g1.Cb = a1; g1.ldc = H;
-        g1.M = rows; g1.N = H; g1.K = fpad; g1.bias = w.p + o_b1; g1.relu = 1;
-        hip_check(smc::gemm(g1, false, false, stream()), "MLP layer 1");
-        GemmArgs g2{};
-        g2.A = a1; g2.lda = H; g2.B = wb.p + o_w2; g2.ldb = H; g2.Cb = a2; g2.ldc = H;
-        g2.M = rows; g2.N = H; g2.K = H; g2.bias = w.p + o_b2; g2.relu = 1;
-        hip_check(smc::gemm(g2, false, false, stream()), "MLP layer 2");
-        GemmArgs g3{};
-        g3.A = a2; g3.lda = H; g3.B = wb.p + o_w3; g3.ldb = Gpad; g3.C = og; g3.ldc = Gpad;
-        g3.M = rows; g3.N = Gpad; g3.K = H; g3.bias = w.p + o_b3; g3.clamp_max = cfg.tilt_max;
-        g3.col_mask = gtiltable.p;
-        hip_check(smc::gemm(g3, false, false, stream()), "MLP layer 3");
-        if (out) hip_check(smc::expand_groups(og, Gpad, group.p, n_locs, Lpad, rows, out, stream()), "expand tilts");
+    // og (rows x ldo; returns ldo: G16 when `compact` and the fused pass
+    // runs, else Gpad), expanded to every location in out (rows x Lpad) when
+    // out is given; a1, a2 keep the bf16 hidden activations for the backward
+    // pass (optional with the fused pass).
+    int forward(const uint16_t *Xin, int rows, uint16_t *a1, uint16_t *a2, float *og, bool compact, float *out) {
+        int ldo = Gpad;
+        if (fused_fwd) {
+            MlpFwdArgs f{};
+            f.X = Xin;
+            f.W1 = wb.p + o_w1; f.W2 = wb.p + o_w2; f.W3 = wb.p + o_w3;
+            f.b1 = w.p + o_b1; f.b2 = w.p + o_b2; f.b3 = w.p + o_b3;
+            f.col_mask = gtiltable.p;
+            f.a1 = a1; f.a2 = a2; f.og = og;
+            f.rows = rows; f.fpad = fpad; f.H = H; f.gpad = Gpad; f.g16 = G16;
+            f.ldo = ldo = compact ? G16 : Gpad;
+            f.clamp_max = cfg.tilt_max;
+            hip_check(smc::mlp_forward(f, stream()), "MLP forward");
+        } else {
+            if (!a1) a1 = h1.p;
+            if (!a2) a2 = h2.p;
+            GemmArgs g1{};
+            g1.A = Xin; g1.lda = fpad; g1.B = wb.p + o_w1; g1.ldb = H; g1.Cb = a1; g1.ldc = H;
+            g1.M = rows; g1.N = H; g1.K = fpad; g1.bias = w.p + o_b1; g1.relu = 1;
+            hip_check(smc::gemm(g1, false, false, stream()), "MLP layer 1");
+            GemmArgs g2{};
+            g2.A = a1; g2.lda = H; g2.B = wb.p + o_w2; g2.ldb = H; g2.Cb = a2; g2.ldc = H;
+            g2.M = rows; g2.N = H; g2.K = H; g2.bias = w.p + o_b2; g2.relu = 1;
+            hip_check(smc::gemm(g2, false, false, stream()), "MLP layer 2");
+            GemmArgs g3{};
+            g3.A = a2; g3.lda = H; g3.B = wb.p + o_w3; g3.ldb = Gpad; g3.C = og; g3.ldc = Gpad;
+            g3.M = rows; g3.N = Gpad; g3.K = H; g3.bias = w.p + o_b3; g3.clamp_max = cfg.tilt_max;
+            g3.col_mask = gtiltable.p;
+            hip_check(smc::gemm(g3, false, false, stream()), "MLP layer 3");
+        }
+        if (out) hip_check(smc::expand_groups(og, ldo, group.p, n_locs, Lpad, rows, out, stream()), "expand tilts");
+        return ldo;
     }
 
     // K splits for the weight gradients (K = rows): about 512 workgroups,
@@ -324,7 +350,7 @@ struct smc_is {
         P.early_stop = b.early_stop ? 1 : 0;
         P.trace = b.trace ? 1 : 0;
         P.stop_at_transition = b.lockstep ? 1 : 0;
-        if (b.tilt == Tilt::FIXED) { P.theta = theta_fixed.p; P.theta_stride = 0; }
+        if (b.tilt == Tilt::FIXED) { P.theta = theta_fixed.p; P.theta_stride = 0; P.tilt_index = tix_loc.p; }
         if (b.record) {
             P.rec_X = rX.p; P.rec_A = rA.p; P.rec_B = rB.p; P.rec_w = rw.p;
             hip_check(hipMemsetAsync(counters.p + 1, 0, sizeof(unsigned int), s), "clear record count");
@@ -334,9 +360,9 @@ struct smc_is {
             if (feats) {
                 // the tilts from the features of the rows' states (the
                 // advance kernel then overwrites the features, in order)
-                forward(A->X.p, rows, h1.p, h2.p, thg.p, theta.p);
-                P.theta = theta.p;
-                P.theta_stride = Lpad;
+                P.theta_stride = forward(A->X.p, rows, nullptr, nullptr, thg.p, true, nullptr);
+                P.theta = thg.p;
+                P.tilt_index = tix_group.p;
             }
             if (b.record) {
                 hip_check(smc::is_record_slots(A->status.p, rows, pos.p, counters.p + 1, (unsigned int)rec_cap, s),
@@ -431,7 +457,7 @@ struct smc_is {
         for (size_t off = 0; off < rec_pairs; off += kTrainChunk) {
             const int rows_c = (int)std::min<size_t>(kTrainChunk, rec_pairs - off);
             const uint16_t *Xc = rX.p + off * fpad;
-            forward(Xc, rows_c, th1.p, th2.p, tthg.p, nullptr);
+            forward(Xc, rows_c, th1.p, th2.p, tthg.p, false, nullptr);
             hip_check(smc::is_ce_grad(tthg.p, rAg.p + off * Gpad, rBg.p + off * Gpad, rw.p + off, gtiltable.p,
                                       cfg.tilt_max, rows_c, Gpad, tthg.p, st), "output gradient");
             hip_check(smc::to_bf16(tthg.p, Gb.p, (size_t)rows_c * Gpad, st), "output gradient to bf16");
@@ -546,6 +572,15 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
             s->n_groups = next;
         }
         s->Gpad = (int)rup((size_t)std::max(s->n_groups, 1), 64);
+        s->G16 = (int)rup((size_t)std::max(s->n_groups, 1), 16);
+        // the fused forward pass when its weights fit in LDS;
+        // SMC_IS_FORWARD=gemm forces the three-GEMM path (the two agree bit
+        // for bit: same products in the same order)
+        {
+            const char *fw = std::getenv("SMC_IS_FORWARD");
+            s->fused_fwd = !(fw && std::string(fw) == "gemm") &&
+                           smc::mlp_forward_supported(s->fpad, s->H, s->G16, s->cfg.tilt_max);
+        }
         const size_t fp = s->fpad, H = s->H, L = s->Lpad, Gp = s->Gpad;
         s->o_w1 = 0;
         s->o_b1 = fp * H;
@@ -586,6 +621,16 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
             for (int l = 0; l < s->n_locs; ++l) mem[fill[s->h_group[l]]++] = l;
         }
         s->gtiltable.ensure(Gp, "hipMalloc tiltable");
+        {
+            std::vector<int32_t> tl(L, -1), tg(L, -1);
+            for (int l = 0; l < s->n_locs; ++l)
+                if (s->h_tiltable[l]) { tl[l] = l; tg[l] = s->h_group[l]; }
+            s->tix_loc.ensure(L, "hipMalloc tilt index");
+            s->tix_group.ensure(L, "hipMalloc tilt index");
+            hip_check(hipMemcpyAsync(s->tix_loc.p, tl.data(), L * 4, hipMemcpyHostToDevice, st), "upload");
+            hip_check(hipMemcpyAsync(s->tix_group.p, tg.data(), L * 4, hipMemcpyHostToDevice, st), "upload");
+            hip_check(hipStreamSynchronize(st), "upload");
+        }
         s->group.ensure(std::max(s->n_locs, 1), "hipMalloc groups");
         s->member_first.ensure(mfirst.size(), "hipMalloc groups");
         s->members.ensure(std::max(s->n_locs, 1), "hipMalloc groups");
@@ -679,7 +724,7 @@ int smc_is_mlp_eval(smc_is *s, const float *x, int rows, const float *gout, floa
         hip_check(hipMemsetAsync(s->xf.p, 0, nx * 4, st), "clear inputs");
         hip_check(hipMemcpyAsync(s->xf.p, x, (size_t)rows * s->fpad * 4, hipMemcpyHostToDevice, st), "upload inputs");
         hip_check(smc::to_bf16(s->xf.p, s->xin.p, nx, st), "inputs to bf16");
-        s->forward(s->xin.p, rp, s->th1.p, s->th2.p, s->tthg.p, s->tth.p);
+        s->forward(s->xin.p, rp, s->th1.p, s->th2.p, s->tthg.p, false, s->tth.p);
         std::vector<float> th((size_t)rows * s->Lpad);
         hip_check(hipMemcpyAsync(th.data(), s->tth.p, th.size() * 4, hipMemcpyDeviceToHost, st), "copy tilts");
         hip_check(hipStreamSynchronize(st), "MLP forward");

@@ -208,8 +208,10 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                                 const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
                                 leaf[j] = r > 0.0 ? r : 0.0;
                                 leaft[j] = leaf[j];
-                                if (theta && leaf[j] > 0.0 && P.tiltable[loc[j]])
-                                    leaft[j] = leaf[j] * smc_exp2((double)theta[loc[j]]);
+                                if (theta && leaf[j] > 0.0) {
+                                    const int ti = P.tilt_index[loc[j]];
+                                    if (ti >= 0) leaft[j] = leaf[j] * smc_exp2((double)theta[ti]);
+                                }
                             }
                         }
                     }
@@ -225,7 +227,10 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                         const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
                         const double lf = r > 0.0 ? r : 0.0;
                         double lt = lf;
-                        if (theta && lf > 0.0 && P.tiltable[lj]) lt = lf * smc_exp2((double)theta[lj]);
+                        if (theta && lf > 0.0) {
+                            const int ti = P.tilt_index[lj];
+                            if (ti >= 0) lt = lf * smc_exp2((double)theta[ti]);
+                        }
                         slot_set<C>(leaf, j, lf);
                         slot_set<C>(leaft, j, lt);
                     }
@@ -324,7 +329,8 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
                             double tw = 0.0;
                             if (lane == wl_lane) {
                                 const int lw = slot_get<C>(loc, wj);
-                                if (P.tiltable[lw]) tw = (double)theta[lw];
+                                const int ti = P.tilt_index[lw];
+                                if (ti >= 0) tw = (double)theta[ti];
                             }
                             th_w = __shfl(tw, wl_lane);
                         }

@@ -2,6 +2,7 @@
 // Adam), row set-up and record-slot kernels of importance sampling, and the
 // launchers of the advance kernel's builds (is_advance.h, is_variant.hip).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 
 #include "is_advance.h"
 
@@ -302,6 +303,138 @@ __global__ void adam_kernel(float *w, const float *gsum, float *m1, float *m2, s
     if (wb) wb[i] = f32_to_bf16(nw);
 }
 
+// ------------------------------------------------------------ fused forward
+//
+// Persistent workgroups of four waves.  The workgroup stages the three
+// weight matrices transposed ([n][k], rows padded by 4 bf16 against bank
+// conflicts) and the biases in LDS once; each wave then takes 16-row tiles:
+// layer 1 reads its A fragments straight from the feature rows, layers 2
+// and 3 from the wave's activation tile in LDS.  MFMA 16x16x16 bf16 with
+// fp32 accumulation, K in steps of 16 from 0 upwards, as gemm_kernel.
+
+__host__ __device__ constexpr size_t mlp_lds_bytes(int fpad, int H, int g16) {
+    return 2 * ((size_t)H * (fpad + 4) + (size_t)H * (H + 4) + (size_t)g16 * (H + 4)) + 4 * (2 * (size_t)H + g16) +
+           2 * (size_t)4 * 16 * (H + 4);
+}
+
+template <int H>
+__global__ __launch_bounds__(256) void mlp_forward_kernel(MlpFwdArgs a) {
+    extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
+    constexpr int NT = H / 16, KP = H + 4;
+    const int KP1 = a.fpad + 4, g16 = a.g16;
+    uint16_t *w1t = smem;
+    uint16_t *w2t = w1t + (size_t)H * KP1;
+    uint16_t *w3t = w2t + (size_t)H * KP;
+    float *b1s = (float *)(w3t + (size_t)g16 * KP);
+    float *b2s = b1s + H;
+    float *b3s = b2s + H;
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    uint16_t *hb = (uint16_t *)(b3s + g16) + (size_t)wave * 16 * KP;
+
+    for (int i = tid; i < a.fpad * H; i += 256) {
+        const int k = i / H, n = i - k * H;
+        w1t[n * KP1 + k] = a.W1[i];
+    }
+    for (int i = tid; i < H * H; i += 256) {
+        const int k = i / H, n = i - k * H;
+        w2t[n * KP + k] = a.W2[i];
+    }
+    for (int i = tid; i < H * g16; i += 256) {
+        const int k = i / g16, n = i - k * g16;
+        w3t[n * KP + k] = a.W3[(size_t)k * a.gpad + n];
+    }
+    for (int i = tid; i < H; i += 256) { b1s[i] = a.b1[i]; b2s[i] = a.b2[i]; }
+    for (int i = tid; i < g16; i += 256) b3s[i] = a.b3[i];
+    __syncthreads();
+
+    const int ar = l & 15, kq = 4 * (l >> 4);  // A/B fragments: row (column) and first k
+    const int cr = 4 * (l >> 4), cc = l & 15;  // C fragment: rows cr..cr+3, column cc
+    const int tiles = a.rows / 16;
+    for (int t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
+        const int r0 = t * 16;
+        f32x4 acc[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // layer 1
+        const uint16_t *xrow = a.X + (size_t)(r0 + ar) * a.fpad + kq;
+        for (int k0 = 0; k0 < a.fpad; k0 += 16) {
+            const bf16x4 fa = *(const bf16x4 *)(xrow + k0);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const bf16x4 fb = *(const bf16x4 *)(w1t + (size_t)(nt * 16 + ar) * KP1 + k0 + kq);
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fb, acc[nt], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int col = nt * 16 + cc;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = acc[nt][i] + b1s[col];
+                v = v > 0.f ? v : 0.f;
+                const uint16_t h = f32_to_bf16(v);
+                hb[(cr + i) * KP + col] = h;
+                if (a.a1) a.a1[(size_t)(r0 + cr + i) * H + col] = h;
+            }
+            acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+        wave_sync();
+        // layer 2
+#pragma unroll 2
+        for (int k0 = 0; k0 < H; k0 += 16) {
+            const bf16x4 fa = *(const bf16x4 *)(hb + ar * KP + k0 + kq);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const bf16x4 fb = *(const bf16x4 *)(w2t + (nt * 16 + ar) * KP + k0 + kq);
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fb, acc[nt], 0, 0, 0);
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int col = nt * 16 + cc;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = acc[nt][i] + b2s[col];
+                v = v > 0.f ? v : 0.f;
+                const uint16_t h = f32_to_bf16(v);
+                hb[(cr + i) * KP + col] = h;
+                if (a.a2) a.a2[(size_t)(r0 + cr + i) * H + col] = h;
+            }
+        }
+        wave_sync();
+        // layer 3 (at most four 16-column tiles)
+        f32x4 o[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) o[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+        for (int k0 = 0; k0 < H; k0 += 16) {
+            const bf16x4 fa = *(const bf16x4 *)(hb + ar * KP + k0 + kq);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                if (nt * 16 < g16) {
+                    const bf16x4 fb = *(const bf16x4 *)(w3t + (nt * 16 + ar) * KP + k0 + kq);
+                    o[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fb, o[nt], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+            if (nt * 16 < g16) {
+                const int col = nt * 16 + cc;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float v = o[nt][i] + b3s[col];
+                    v = v > a.clamp_max ? a.clamp_max : (v < -a.clamp_max ? -a.clamp_max : v);
+                    if (!a.col_mask[col]) v = 0.f;
+                    a.og[(size_t)(r0 + cr + i) * a.ldo + col] = v;
+                }
+            }
+        for (int c = g16 + l; c < a.ldo; c += 64)
+            for (int r = 0; r < 16; ++r) a.og[(size_t)(r0 + r) * a.ldo + c] = 0.f;
+        wave_sync();  // the next tile overwrites hb
+    }
+}
+
 }  // namespace
 
 namespace smc {
@@ -329,6 +462,46 @@ static hipError_t gemm_launch(const GemmArgs &g, bool ta, bool tb, int zsplit, h
     else if (ta && !tb) hipLaunchKernelGGL((gemm_kernel<true, false>), grid, dim3(256), 0, s, g);
     else if (!ta && tb) hipLaunchKernelGGL((gemm_kernel<false, true>), grid, dim3(256), 0, s, g);
     else hipLaunchKernelGGL((gemm_kernel<true, true>), grid, dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+static int max_lds_per_block() {
+    static int v = -1;
+    if (v < 0) {
+        int dev = 0, x = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&x, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
+            v = x;
+        else
+            v = 0;
+    }
+    return v;
+}
+
+bool mlp_forward_supported(int fpad, int H, int g16, float clamp_max) {
+    if (!(H == 64 || H == 128 || H == 256) || fpad <= 0 || fpad % 16 || g16 <= 0 || g16 % 16 || g16 > 64 ||
+        !(clamp_max > 0.f))
+        return false;
+    return mlp_lds_bytes(fpad, H, g16) <= (size_t)max_lds_per_block();
+}
+
+hipError_t mlp_forward(const MlpFwdArgs &a, hipStream_t s) {
+    if (a.rows <= 0 || a.rows % 16 || a.g16 > a.gpad || a.ldo < a.g16 || !a.og || !a.X ||
+        !mlp_forward_supported(a.fpad, a.H, a.g16, a.clamp_max))
+        return hipErrorInvalidValue;
+    const size_t lds = mlp_lds_bytes(a.fpad, a.H, a.g16);
+    const void *fn = a.H == 64 ? (const void *)mlp_forward_kernel<64>
+                   : a.H == 128 ? (const void *)mlp_forward_kernel<128> : (const void *)mlp_forward_kernel<256>;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int dev = 0, cus = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    const int tiles = a.rows / 16;
+    const int grid = std::max(1, std::min((tiles + 3) / 4, cus));
+    if (a.H == 64) hipLaunchKernelGGL(mlp_forward_kernel<64>, dim3(grid), dim3(256), lds, s, a);
+    else if (a.H == 128) hipLaunchKernelGGL(mlp_forward_kernel<128>, dim3(grid), dim3(256), lds, s, a);
+    else hipLaunchKernelGGL(mlp_forward_kernel<256>, dim3(grid), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 

@@ -48,6 +48,7 @@ struct IsParams {
     const float *theta;         // log2 tilts per location, or null (no tilt)
     int32_t theta_stride;       // row stride of theta (0: one row for every run)
     const uint8_t *tiltable;    // [lpad]
+    const int32_t *tilt_index;  // [lpad]: the location's column in a theta row, -1 if untiltable
     uint16_t *feat;             // [rows][fpad] bf16 features of the end state, or null
     const float *var_scale;     // [n_vars]
     // recording of cross-entropy statistics (replay of elite runs)
@@ -58,6 +59,21 @@ struct IsParams {
     float *rec_w;               // [slots] weight of the row
     const float *weight;        // [jobs]
     unsigned int *active_count; // rows still active after the launch
+};
+
+// The proposal network's forward pass in one launch: theta_g =
+// clamp(W3^T relu(W2^T relu(W1^T x + b1) + b2) + b3), zero at untiltable
+// groups.  Same products, same fp32 accumulation order and epilogues as
+// three chained gemm() calls, so the outputs are bit-identical to them.
+struct MlpFwdArgs {
+    const uint16_t *X;                 // [rows][fpad] bf16 features
+    const uint16_t *W1, *W2, *W3;      // bf16 row-major [fpad][H], [H][H], [H][gpad]
+    const float *b1, *b2, *b3;         // [H], [H], [gpad]
+    const uint8_t *col_mask;           // [gpad]: group tiltable
+    uint16_t *a1, *a2;                 // optional [rows][H] bf16 activations (for the backward pass)
+    float *og;                         // [rows][ldo]: outputs g < g16, zeros g16 <= g < ldo
+    int rows, fpad, H, gpad, g16, ldo; // rows % 16 == 0; g16 = groups rounded up to 16 (<= 64)
+    float clamp_max;                   // > 0
 };
 
 struct GemmArgs {
@@ -118,6 +134,9 @@ hipError_t expand_groups(const float *tg, int gpad, const int32_t *group, int n_
 hipError_t group_sum(const float *gl, int lpad, const int32_t *member_first, const int32_t *members, int n_groups,
                      int gpad, int rows, float *gg, uint16_t *ggb, hipStream_t s);
 hipError_t gemm(const GemmArgs &g, bool ta, bool tb, hipStream_t s);
+// Whether mlp_forward handles these sizes (H 64/128/256, weights in LDS).
+bool mlp_forward_supported(int fpad, int H, int g16, float clamp_max);
+hipError_t mlp_forward(const MlpFwdArgs &a, hipStream_t s);
 // dst (+)= op(A) op(B) with K split over `splits` workgroup layers (partial
 // sums in `work`, splits * M * N floats, folded in a fixed order).
 hipError_t gemm_splitk(const GemmArgs &g, bool ta, bool tb, int splits, float *work, hipStream_t s);
