@@ -131,6 +131,47 @@ __device__ __forceinline__ T load_rec(const SMC_LDS T *p) {
 }
 #endif
 
+// Table pointer of the global arena rebased onto its LDS copy.
+template <class T>
+__device__ __forceinline__ const SMC_LDS T *in_lds(const SMC_GLOBAL T *p, const SMC_GLOBAL uint8_t *base,
+                                                   const SMC_LDS uint8_t *copy) {
+    return (const SMC_LDS T *)(copy + ((const SMC_GLOBAL uint8_t *)p - base));
+}
+
+// Copy the arena prefix [0, bytes) into the start of the workgroup's LDS
+// (every thread of the workgroup takes part; ends with a barrier).
+__device__ __forceinline__ void stage_tables(const SMC_GLOBAL uint8_t *src, uint32_t bytes, double *lds) {
+    const SMC_GLOBAL uint4 *s4 = (const SMC_GLOBAL uint4 *)src;
+    SMC_LDS uint4 *d4 = (SMC_LDS uint4 *)(SMC_LDS double *)lds;
+    for (uint32_t i = threadIdx.x; i < bytes / 16; i += blockDim.x) d4[i] = s4[i];
+    __syncthreads();
+}
+
+// The model with its hot tables read from the staged copy.
+__device__ __forceinline__ DevModelT<SmcLdsTables> lds_model(const DevModel &mg, const SMC_GLOBAL uint8_t *base,
+                                                             double *lds) {
+    const SMC_LDS uint8_t *tab = (const SMC_LDS uint8_t *)(SMC_LDS double *)lds;
+    DevModelT<SmcLdsTables> m;
+    m.locs = in_lds(mg.locs, base, tab);
+    m.invs = in_lds(mg.invs, base, tab);
+    m.edges = in_lds(mg.edges, base, tab);
+    m.guards = in_lds(mg.guards, base, tab);
+    m.upds = in_lds(mg.upds, base, tab);
+    m.consts = in_lds(mg.consts, base, tab);
+    m.code = in_lds(mg.code, base, tab);
+    m.var_init = mg.var_init;
+    m.comp_init_loc = mg.comp_init_loc;
+    m.clocks = mg.clocks;
+    m.comp_first_loc = mg.comp_first_loc;
+    m.n_comp = mg.n_comp;
+    m.n_vars = mg.n_vars;
+    m.n_clocks = mg.n_clocks;
+    m.n_leaves = mg.n_leaves;
+    m.n_locs = mg.n_locs;
+    m.n_edges = mg.n_edges;
+    return m;
+}
+
 // ------------------------------------------------------------ expressions
 
 // Register-bytecode interpreter (expressions.cu:10-119).  Inlined: an

@@ -140,11 +140,16 @@ struct smc_is {
     smc_is_config cfg{};
     int S = 1, C = 1, R = 64, stride = 1, occ = 0;
     size_t lds = 0;
+    uint32_t wave_lds = 0;
+    bool tab = false;            // advance kernel stages the model tables in LDS
+    int max_groups = 0;          // its persistent workgroups when it does
     int n_locs = 0, n_vars = 0, n_comp = 0, F = 0, fpad = 0, H = 128, Lpad = 0;
     int n_groups = 0, Gpad = 0;  // tilt groups: the MLP's outputs
     int G16 = 0;                 // groups rounded up to 16: the outputs the fused forward pass computes
     bool fused_fwd = false;      // forward pass in one kernel (weights fit in LDS)
     DevBuf<int32_t> tix_loc, tix_group;  // per location: its theta column (per location / per group), -1 if untiltable
+    DevBuf<uint8_t> mlp_img;     // weights packed as the fused pass stages them in LDS
+    bool packed = false;         // mlp_img matches the current weights
     size_t o_w1 = 0, o_b1 = 0, o_w2 = 0, o_b2 = 0, o_w3 = 0, o_b3 = 0, n_params = 0;
     double qinit = 0.0;
     std::vector<uint8_t> h_tiltable;
@@ -235,6 +240,9 @@ struct smc_is {
         P.max_steps = q.max_steps ? q.max_steps : SMC_DEFAULT_MAX_STEPS;
         P.tiltable = tiltable.p;
         P.tilt_index = tix_loc.p;
+        P.tables = tab ? (decltype(P.tables))m->arena : nullptr;
+        P.table_bytes = tab ? m->table_bytes : 0;
+        P.wave_lds = wave_lds;
         P.var_scale = var_scale.p;
         P.weight = weight.p;
         P.active_count = counters.p;
@@ -249,10 +257,15 @@ struct smc_is {
     int forward(const uint16_t *Xin, int rows, uint16_t *a1, uint16_t *a2, float *og, bool compact, float *out) {
         int ldo = Gpad;
         if (fused_fwd) {
+            if (!packed) {
+                mlp_img.ensure(smc::mlp_image_bytes(fpad, H, G16), "hipMalloc packed weights");
+                hip_check(smc::mlp_pack(w.p, o_w1, o_b1, o_w2, o_b2, o_w3, o_b3, fpad, H, Gpad, G16, mlp_img.p,
+                                        stream()), "pack weights");
+                packed = true;
+            }
             MlpFwdArgs f{};
             f.X = Xin;
-            f.W1 = wb.p + o_w1; f.W2 = wb.p + o_w2; f.W3 = wb.p + o_w3;
-            f.b1 = w.p + o_b1; f.b2 = w.p + o_b2; f.b3 = w.p + o_b3;
+            f.image = mlp_img.p;
             f.col_mask = gtiltable.p;
             f.a1 = a1; f.a2 = a2; f.og = og;
             f.rows = rows; f.fpad = fpad; f.H = H; f.gpad = Gpad; f.g16 = G16;
@@ -370,7 +383,7 @@ struct smc_is {
                 P.rec_pos = pos.p;
             }
             hip_check(hipMemsetAsync(counters.p, 0, sizeof(unsigned int), s), "clear active count");
-            hip_check(smc::is_launch_advance(P, C, occ, lds, s), "advance kernel launch");
+            hip_check(smc::is_launch_advance(P, C, occ, lds, max_groups, s), "advance kernel launch");
             ++launches;
             if (!b.lockstep) break;
             hip_check(hipMemcpyAsync(h_counters, counters.p, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, s),
@@ -474,6 +487,7 @@ struct smc_is {
     void upload_weights(const std::vector<float> &hw) {
         hip_check(hipMemcpyAsync(w.p, hw.data(), n_params * 4, hipMemcpyHostToDevice, stream()), "upload weights");
         hip_check(smc::to_bf16(w.p, wb.p, n_params, stream()), "weights to bf16");
+        packed = false;
         hip_check(hipStreamSynchronize(stream()), "upload weights");
     }
 };
@@ -521,8 +535,20 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         if (s->C > 16) throw std::runtime_error("limit: more than 1024 components");
         s->R = 64 / s->S;
         s->stride = (img.n_vars > 0 ? img.n_vars : 1) | 1;
-        s->lds = (size_t)s->R * s->stride * sizeof(double);
-        if (s->lds > 60000) throw std::runtime_error("limit: variables exceed the LDS budget");
+        s->wave_lds = (uint32_t)rup((size_t)s->R * s->stride * sizeof(double), 16);
+        if (s->wave_lds > 60000) throw std::runtime_error("limit: variables exceed the LDS budget");
+        // hot tables staged in LDS when the model's tuned launch does so (or
+        // SMC_IS_TABLES=1/0 says) and four waves' variables fit beside them
+        {
+            const char *tv = std::getenv("SMC_IS_TABLES");
+            bool want = m->has_tuned && m->tuned.tables == 1;
+            if (tv) want = std::string(tv) == "1";
+            const size_t need = (size_t)m->table_bytes + 4 * (size_t)s->wave_lds;
+            s->tab = want && m->table_bytes > 0 && need <= 160 * 1024;
+            s->lds = s->tab ? need : s->wave_lds;
+            const int per_cu = s->tab ? std::max(1, std::min(s->occ == 2 ? 2 : 1, (int)((160 * 1024) / need))) : 0;
+            s->max_groups = per_cu * m->cu_count;
+        }
 
         s->n_locs = (int)img.locs.size();
         s->n_vars = img.n_vars;
@@ -944,6 +970,7 @@ int smc_is_train(smc_is *s, smc_is_round *rounds) {
                     s->ce_gradient();
                     hip_check(smc::adam_step(s->w.p, s->g.p, s->m1.p, s->m2.p, s->n_params, c.learning_rate,
                                              ++s->adam_t, s->wb.p, st), "Adam step");
+                    s->packed = false;
                     s->zero_output = false;
                 }
                 hip_check(hipStreamSynchronize(st), "training");

@@ -80,18 +80,16 @@ __device__ __forceinline__ bool settled(const LaneQuery &q, const IsParams &P, i
 
 // ------------------------------------------------------------ advance
 
-template <int C>
-__device__ __forceinline__ void is_advance_body(const IsParams &P) {
-    extern __shared__ double lds[];
-    const int lane = threadIdx.x;
+template <int C, class MT>
+__device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, double *wlds, int group) {
+    const int lane = threadIdx.x & 63;
     const int S = P.lanes;
     const int sl = lane & (S - 1);
     const int seg_base = lane - sl;
     const uint64_t seg_mask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << seg_base);
-    const int row = blockIdx.x * (64 / S) + lane / S;
+    const int row = group * (64 / S) + lane / S;
     if (row >= P.n_rows) return;  // whole segments only
-    double *vars = lds + (size_t)(lane / S) * P.var_stride;
-    const DevModel &m = P.m;
+    double *vars = wlds + (size_t)(lane / S) * P.var_stride;
     const IsRows &st = P.st;
     const int n = m.n_comp;
     const double INF = __builtin_huge_val();
@@ -472,16 +470,42 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
     }
 }
 
-// Registers unrestricted, or at most 256 (two waves per SIMD); see the run
-// kernel.
-template <int C>
-__global__ __launch_bounds__(64) void is_advance_kernel(IsParams P) {
-    is_advance_body<C>(P);
+// A workgroup holds blockDim/64 waves; each takes groups of 64/S rows
+// (blockIdx.x * waves + wave, then strided by the grid).  With TAB the
+// workgroup first stages the model's hot tables in LDS (as the run kernel).
+// LDS: [tables: TAB only][per wave: variables of its rows].
+template <int C, bool TAB>
+__device__ __forceinline__ void is_advance_body(const IsParams &P) {
+    extern __shared__ __attribute__((aligned(16))) double lds[];
+    const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
+    double *wlds = lds + (TAB ? P.table_bytes / 8 : 0) + (size_t)wv * (P.wave_lds / 8);
+    const int R = 64 / P.lanes;
+    const int groups = (P.n_rows + R - 1) / R;
+    if constexpr (TAB) {
+        stage_tables(P.tables, P.table_bytes, lds);
+        const DevModelT<SmcLdsTables> m = lds_model(P.m, P.tables, lds);
+        for (int g = blockIdx.x * W + wv; g < groups; g += gridDim.x * W) {
+            is_advance_rows<C>(P, m, wlds, g);
+            wave_sync();
+        }
+    } else {
+        for (int g = blockIdx.x * W + wv; g < groups; g += gridDim.x * W) {
+            is_advance_rows<C>(P, P.m, wlds, g);
+            wave_sync();
+        }
+    }
 }
 
-template <int C>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void is_advance_kernel_o2(IsParams P) {
-    is_advance_body<C>(P);
+// Registers unrestricted, or at most 256 (two waves per SIMD); see the run
+// kernel.
+template <int C, bool TAB>
+__global__ __launch_bounds__(256) void is_advance_kernel(IsParams P) {
+    is_advance_body<C, TAB>(P);
+}
+
+template <int C, bool TAB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void is_advance_kernel_o2(IsParams P) {
+    is_advance_body<C, TAB>(P);
 }
 
 }  // namespace

@@ -312,9 +312,36 @@ __global__ void adam_kernel(float *w, const float *gsum, float *m1, float *m2, s
 // and 3 from the wave's activation tile in LDS.  MFMA 16x16x16 bf16 with
 // fp32 accumulation, K in steps of 16 from 0 upwards, as gemm_kernel.
 
+__host__ __device__ constexpr size_t mlp_image_size(int fpad, int H, int g16) {
+    return 2 * ((size_t)H * (fpad + 4) + (size_t)H * (H + 4) + (size_t)g16 * (H + 4)) + 4 * (2 * (size_t)H + g16);
+}
+
 __host__ __device__ constexpr size_t mlp_lds_bytes(int fpad, int H, int g16) {
-    return 2 * ((size_t)H * (fpad + 4) + (size_t)H * (H + 4) + (size_t)g16 * (H + 4)) + 4 * (2 * (size_t)H + g16) +
-           2 * (size_t)4 * 16 * (H + 4);
+    return mlp_image_size(fpad, H, g16) + 2 * (size_t)4 * 16 * (H + 4);
+}
+
+// The packed image, one thread per element: bf16 part, then fp32 biases.
+__global__ void mlp_pack_kernel(const float *w, size_t o_w1, size_t o_b1, size_t o_w2, size_t o_b2, size_t o_w3,
+                                size_t o_b3, int fpad, int H, int gpad, int g16, uint16_t *img) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t KP1 = fpad + 4, KP = H + 4;
+    const size_t n1 = (size_t)H * KP1, n2 = (size_t)H * KP, n3 = (size_t)g16 * KP;
+    if (i < n1) {
+        const size_t n = i / KP1, k = i % KP1;
+        img[i] = k < (size_t)fpad ? f32_to_bf16(w[o_w1 + k * H + n]) : 0;
+    } else if (i < n1 + n2) {
+        const size_t j = i - n1, n = j / KP, k = j % KP;
+        img[i] = k < (size_t)H ? f32_to_bf16(w[o_w2 + k * H + n]) : 0;
+    } else if (i < n1 + n2 + n3) {
+        const size_t j = i - n1 - n2, n = j / KP, k = j % KP;
+        img[i] = k < (size_t)H ? f32_to_bf16(w[o_w3 + k * gpad + n]) : 0;
+    } else {
+        const size_t j = i - n1 - n2 - n3;
+        float *b = (float *)(img + n1 + n2 + n3);
+        if (j < (size_t)H) b[j] = w[o_b1 + j];
+        else if (j < 2 * (size_t)H) b[j] = w[o_b2 + (j - H)];
+        else if (j < 2 * (size_t)H + g16) b[j] = w[o_b3 + (j - 2 * H)];
+    }
 }
 
 template <int H>
@@ -331,20 +358,12 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpFwdArgs a) {
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
     uint16_t *hb = (uint16_t *)(b3s + g16) + (size_t)wave * 16 * KP;
 
-    for (int i = tid; i < a.fpad * H; i += 256) {
-        const int k = i / H, n = i - k * H;
-        w1t[n * KP1 + k] = a.W1[i];
+    {
+        const uint4 *src = (const uint4 *)a.image;
+        uint4 *dst = (uint4 *)smem;
+        const int n16 = (int)(mlp_image_size(a.fpad, H, g16) / 16);
+        for (int i = tid; i < n16; i += 256) dst[i] = src[i];
     }
-    for (int i = tid; i < H * H; i += 256) {
-        const int k = i / H, n = i - k * H;
-        w2t[n * KP + k] = a.W2[i];
-    }
-    for (int i = tid; i < H * g16; i += 256) {
-        const int k = i / g16, n = i - k * g16;
-        w3t[n * KP + k] = a.W3[(size_t)k * a.gpad + n];
-    }
-    for (int i = tid; i < H; i += 256) { b1s[i] = a.b1[i]; b2s[i] = a.b2[i]; }
-    for (int i = tid; i < g16; i += 256) b3s[i] = a.b3[i];
     __syncthreads();
 
     const int ar = l & 15, kq = 4 * (l >> 4);  // A/B fragments: row (column) and first k
@@ -439,10 +458,16 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpFwdArgs a) {
 
 namespace smc {
 
-hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, hipStream_t s) {
+hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, int max_groups, hipStream_t s) {
     const int R = 64 / p.lanes;
-    const int grid = (p.n_rows + R - 1) / R;
-    if (grid <= 0) return hipSuccess;
+    const int row_groups = (p.n_rows + R - 1) / R;
+    if (row_groups <= 0) return hipSuccess;
+    // staged tables: workgroups of four waves, at most max_groups of them
+    // (persistent: each wave loops over row groups); else one wave per
+    // workgroup and one row group per wave
+    const int W = p.table_bytes ? 4 : 1;
+    int grid = (row_groups + W - 1) / W;
+    if (p.table_bytes && max_groups > 0 && grid > max_groups) grid = max_groups;
     IsAdvanceLaunch f = nullptr;
     const bool o2 = occ == 2;
     switch (slots) {
@@ -453,7 +478,7 @@ hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, 
     case 16: f = o2 ? is_advance_c16_o2() : is_advance_c16_o0(); break;
     default: return hipErrorInvalidValue;
     }
-    return f(p, grid, lds, s);
+    return f(p, grid, W, lds, s);
 }
 
 static hipError_t gemm_launch(const GemmArgs &g, bool ta, bool tb, int zsplit, hipStream_t s) {
@@ -485,8 +510,18 @@ bool mlp_forward_supported(int fpad, int H, int g16, float clamp_max) {
     return mlp_lds_bytes(fpad, H, g16) <= (size_t)max_lds_per_block();
 }
 
+size_t mlp_image_bytes(int fpad, int H, int g16) { return mlp_image_size(fpad, H, g16); }
+
+hipError_t mlp_pack(const float *w, size_t o_w1, size_t o_b1, size_t o_w2, size_t o_b2, size_t o_w3, size_t o_b3,
+                    int fpad, int H, int gpad, int g16, void *image, hipStream_t s) {
+    const size_t n = (size_t)H * (fpad + 4) + (size_t)H * (H + 4) + (size_t)g16 * (H + 4) + 2 * (size_t)H + g16;
+    hipLaunchKernelGGL(mlp_pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w, o_w1, o_b1, o_w2, o_b2,
+                       o_w3, o_b3, fpad, H, gpad, g16, (uint16_t *)image);
+    return hipGetLastError();
+}
+
 hipError_t mlp_forward(const MlpFwdArgs &a, hipStream_t s) {
-    if (a.rows <= 0 || a.rows % 16 || a.g16 > a.gpad || a.ldo < a.g16 || !a.og || !a.X ||
+    if (a.rows <= 0 || a.rows % 16 || a.g16 > a.gpad || a.ldo < a.g16 || !a.og || !a.X || !a.image ||
         !mlp_forward_supported(a.fpad, a.H, a.g16, a.clamp_max))
         return hipErrorInvalidValue;
     const size_t lds = mlp_lds_bytes(a.fpad, a.H, a.g16);

@@ -49,6 +49,9 @@ struct IsParams {
     int32_t theta_stride;       // row stride of theta (0: one row for every run)
     const uint8_t *tiltable;    // [lpad]
     const int32_t *tilt_index;  // [lpad]: the location's column in a theta row, -1 if untiltable
+    const SMC_GLOBAL uint8_t *tables;  // hot tables staged in LDS when table_bytes > 0: arena start
+    uint32_t table_bytes;       // bytes staged (multiple of 16)
+    uint32_t wave_lds;          // LDS bytes of one wave after the tables (multiple of 16)
     uint16_t *feat;             // [rows][fpad] bf16 features of the end state, or null
     const float *var_scale;     // [n_vars]
     // recording of cross-entropy statistics (replay of elite runs)
@@ -67,8 +70,7 @@ struct IsParams {
 // three chained gemm() calls, so the outputs are bit-identical to them.
 struct MlpFwdArgs {
     const uint16_t *X;                 // [rows][fpad] bf16 features
-    const uint16_t *W1, *W2, *W3;      // bf16 row-major [fpad][H], [H][H], [H][gpad]
-    const float *b1, *b2, *b3;         // [H], [H], [gpad]
+    const void *image;                 // the weights as staged in LDS (mlp_pack)
     const uint8_t *col_mask;           // [gpad]: group tiltable
     uint16_t *a1, *a2;                 // optional [rows][H] bf16 activations (for the backward pass)
     float *og;                         // [rows][ldo]: outputs g < g16, zeros g16 <= g < ldo
@@ -95,7 +97,7 @@ struct GemmArgs {
 namespace smc {
 
 // One build of the advance kernel (is_variant.hip).
-typedef hipError_t (*IsAdvanceLaunch)(const IsParams &p, int grid, size_t lds, hipStream_t s);
+typedef hipError_t (*IsAdvanceLaunch)(const IsParams &p, int grid, int group_waves, size_t lds, hipStream_t s);
 IsAdvanceLaunch is_advance_c1_o0();
 IsAdvanceLaunch is_advance_c2_o0();
 IsAdvanceLaunch is_advance_c4_o0();
@@ -108,7 +110,7 @@ IsAdvanceLaunch is_advance_c8_o2();
 IsAdvanceLaunch is_advance_c16_o2();
 
 // occ: 0 registers unrestricted, 2 at most 256 (two waves per SIMD)
-hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, hipStream_t s);
+hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, int max_groups, hipStream_t s);
 // Reset rows [0, rows) to the initial state of runs runs[i] (or run_begin + i);
 // rows >= n_valid are inactive.  feat0 is the feature row of the initial state.
 hipError_t is_init_rows(const IsParams &p, const uint64_t *runs, uint64_t run_begin, int n_valid,
@@ -136,6 +138,13 @@ hipError_t group_sum(const float *gl, int lpad, const int32_t *member_first, con
 hipError_t gemm(const GemmArgs &g, bool ta, bool tb, hipStream_t s);
 // Whether mlp_forward handles these sizes (H 64/128/256, weights in LDS).
 bool mlp_forward_supported(int fpad, int H, int g16, float clamp_max);
+// Bytes of the packed weight image: W1, W2, W3 transposed to [n][k] bf16
+// (rows padded by 4), then b1, b2, b3 (first g16) in fp32.
+size_t mlp_image_bytes(int fpad, int H, int g16);
+// Pack it from the flat fp32 parameters (offsets of W1, b1, W2, b2, W3, b3);
+// bf16 rounding as to_bf16, so the fused pass multiplies the same values.
+hipError_t mlp_pack(const float *w, size_t o_w1, size_t o_b1, size_t o_w2, size_t o_b2, size_t o_w3, size_t o_b3,
+                    int fpad, int H, int gpad, int g16, void *image, hipStream_t s);
 hipError_t mlp_forward(const MlpFwdArgs &a, hipStream_t s);
 // dst (+)= op(A) op(B) with K split over `splits` workgroup layers (partial
 // sums in `work`, splits * M * N floats, folded in a fixed order).

@@ -1,6 +1,7 @@
 // is_variant.hip — one build of the tilted advance kernel (is_advance.h):
 // SMC_C component slots per lane, register policy SMC_OCC (0 unrestricted,
-// 2 at most 256 registers).  The build compiles this file once per pair.
+// 2 at most 256 registers), with the tables in global memory or staged in
+// LDS.  The build compiles this file once per pair.
 #include "is_advance.h"
 
 #if !defined(SMC_C) || !defined(SMC_OCC)
@@ -12,11 +13,14 @@
 
 namespace {
 
-hipError_t launch(const IsParams &p, int grid, size_t lds, hipStream_t s) {
+hipError_t launch(const IsParams &p, int grid, int group_waves, size_t lds, hipStream_t s) {
+    const dim3 g(grid), b(64 * group_waves);
 #if SMC_OCC == 2
-    hipLaunchKernelGGL(is_advance_kernel_o2<SMC_C>, dim3(grid), dim3(64), lds, s, p);
+    if (p.table_bytes) hipLaunchKernelGGL((is_advance_kernel_o2<SMC_C, true>), g, b, lds, s, p);
+    else hipLaunchKernelGGL((is_advance_kernel_o2<SMC_C, false>), g, b, lds, s, p);
 #else
-    hipLaunchKernelGGL(is_advance_kernel<SMC_C>, dim3(grid), dim3(64), lds, s, p);
+    if (p.table_bytes) hipLaunchKernelGGL((is_advance_kernel<SMC_C, true>), g, b, lds, s, p);
+    else hipLaunchKernelGGL((is_advance_kernel<SMC_C, false>), g, b, lds, s, p);
 #endif
     return hipGetLastError();
 }

@@ -49,13 +49,6 @@ namespace {
 // ------------------------------------------------------------ run kernel
 
 
-// Table pointer of the global arena rebased onto its LDS copy.
-template <class T>
-__device__ __forceinline__ const SMC_LDS T *in_lds(const SMC_GLOBAL T *p, const SMC_GLOBAL uint8_t *base,
-                                                   const SMC_LDS uint8_t *copy) {
-    return (const SMC_LDS T *)(copy + ((const SMC_GLOBAL uint8_t *)p - base));
-}
-
 // One wave at a time takes a chunk of runs of one instance from the launch's
 // queue, so the instance (its tables, query and outputs) is uniform across
 // the wave and lives in scalar registers; the wave's segments then pull the
@@ -89,13 +82,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
     const double INF = __builtin_huge_val();
     const uint64_t round = lp.round ? *lp.round : 0;
     using Tables = typename std::conditional<TAB, SmcLdsTables, SmcGlobalTables>::type;
-    const SMC_LDS uint8_t *tab = (const SMC_LDS uint8_t *)(SMC_LDS double *)lds;
-    if (TAB) {
-        const SMC_GLOBAL uint4 *src = (const SMC_GLOBAL uint4 *)lp.tables;
-        SMC_LDS uint4 *dst = (SMC_LDS uint4 *)(SMC_LDS double *)lds;
-        for (uint32_t i = threadIdx.x; i < lp.table_bytes / 16; i += blockDim.x) dst[i] = src[i];
-        __syncthreads();
-    }
+    if (TAB) stage_tables(lp.tables, lp.table_bytes, lds);
 
   for (;;) {
     unsigned long long g = 0;
@@ -110,27 +97,8 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
     }
     const DevModel mg = lp.inst[lo].m;
     DevModelT<Tables> m;
-    if constexpr (TAB) {
-        m.locs = in_lds(mg.locs, lp.tables, tab);
-        m.invs = in_lds(mg.invs, lp.tables, tab);
-        m.edges = in_lds(mg.edges, lp.tables, tab);
-        m.guards = in_lds(mg.guards, lp.tables, tab);
-        m.upds = in_lds(mg.upds, lp.tables, tab);
-        m.consts = in_lds(mg.consts, lp.tables, tab);
-        m.code = in_lds(mg.code, lp.tables, tab);
-        m.var_init = mg.var_init;
-        m.comp_init_loc = mg.comp_init_loc;
-        m.clocks = mg.clocks;
-        m.comp_first_loc = mg.comp_first_loc;
-        m.n_comp = mg.n_comp;
-        m.n_vars = mg.n_vars;
-        m.n_clocks = mg.n_clocks;
-        m.n_leaves = mg.n_leaves;
-        m.n_locs = mg.n_locs;
-        m.n_edges = mg.n_edges;
-    } else {
-        m = mg;
-    }
+    if constexpr (TAB) m = lds_model(mg, lp.tables, lds);
+    else m = mg;
     const SimParams p = lp.inst[lo].p;
     const uint64_t chunk = g - lp.inst[lo].chunk_first;
     const uint32_t r_first = (uint32_t)(chunk * lp.chunk_runs);

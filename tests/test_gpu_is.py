@@ -105,6 +105,32 @@ def test_fixed_tilt_matches_oracle(env, name):
         assert np.array_equal(r.per_run["time"], o["time"])
 
 
+@pytest.mark.parametrize("name", sorted(MODELS))
+def test_staged_tables_match_oracle(env, name, monkeypatch):
+    """The advance kernel with the model tables staged in LDS (persistent
+    four-wave workgroups): fixed-tilt runs equal the oracle's bit for bit,
+    recorded pairs too."""
+    net, q, sim, _ = env[name]
+    monkeypatch.setenv("SMC_IS_TABLES", "1")
+    s = cs.ImportanceSampler(sim, q, runs_per_round=4096, rounds=1, epochs=2, seed=3)
+    monkeypatch.delenv("SMC_IS_TABLES")
+    try:
+        theta = _theta(net, 5)
+        n = 1500
+        for lock, early in ((False, False), (True, True)):
+            print(f"[is staged tables] {name} lockstep={lock} early_stop={early}", flush=True)
+            r = s.sample(seed=9, n_runs=n, theta=theta, lockstep=lock, early_stop=early)
+            o = oracle.is_run(net, q, seed=9, n_runs=n, theta=theta, early_stop=early, threads=0)
+            for k in ("digest", "flag", "log2lr", "steps", "time"):
+                assert np.array_equal(r.per_run[k], o[k]), (k, lock, early)
+        runs = np.arange(100, 108, dtype=np.uint64)
+        X, A, B = s.record(runs, seed=9, theta=theta, early_stop=True)
+        X0, A0, B0 = env[name][3].record(runs, seed=9, theta=theta, early_stop=True)
+        assert np.array_equal(X, X0) and np.array_equal(A, A0) and np.array_equal(B, B0)
+    finally:
+        s.close()
+
+
 @pytest.mark.parametrize("name", ["sir30", "fischer6", "race8"])
 def test_recorded_pairs_match_oracle(env, name):
     net, q, sim, s = env[name]
