@@ -209,6 +209,27 @@ __device__ __forceinline__ double eval_ref(uint32_t ref, const double *vars, con
     return eval_code(ref, vars, m.consts, m.code);
 }
 
+// A location's rate within one race, remembering the last expression this
+// lane evaluated: identical expressions share one reference (the compiler
+// deduplicates bytecode), and variables do not change during a race, so a
+// repeated reference has the same value.  Components of one template in the
+// same location (every susceptible agent of an epidemic, say) then cost one
+// evaluation per lane instead of one per slot.
+struct RateMemo {
+    uint32_t ref = 0xFFFFFFFFu;
+    double value = 0.0;
+};
+
+template <class M>
+__device__ __forceinline__ double location_rate(const DevLoc &L, const double *vars, const M &m, RateMemo &memo) {
+    if (L.rate_const) return L.rate_value;
+    if (L.rate != memo.ref) {
+        memo.value = eval_ref(L.rate, vars, m);
+        memo.ref = L.rate;
+    }
+    return memo.value;
+}
+
 // ------------------------------------------------------------ edges
 
 __device__ __forceinline__ bool guard_holds(int op, double x, double r) {
@@ -283,12 +304,13 @@ template <int C, class M>
 __device__ __forceinline__ double dead_total(uint32_t deadmask, const int (&loc)[C], const double *vars,
                                              const M &m, int S) {
     double dl[C];
+    RateMemo memo;
 #pragma unroll
     for (int j = 0; j < C; ++j) {
         dl[j] = 0.0;
         if ((deadmask >> j) & 1u) {
             const DevLoc L = load_rec(m.locs + loc[j]);
-            const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
+            const double r = location_rate(L, vars, m, memo);
             dl[j] = r > 0.0 ? r : 0.0;
         }
     }
@@ -330,6 +352,7 @@ template <int C, class M>
 __device__ __forceinline__ bool absorbed(const int (&loc)[C], const double *vars, const M &m, int sl, int n,
                                          uint64_t seg_mask) {
     bool can = false;
+    RateMemo memo;
 #pragma unroll
     for (int j = 0; j < C; ++j) {
         const int c = sl * C + j;
@@ -338,7 +361,7 @@ __device__ __forceinline__ bool absorbed(const int (&loc)[C], const double *vars
         if (!(L.flags & SMC_LF_TIME_FREE) || L.klass == SMC_K_ZERO) { can = true; continue; }
         if (enabled_mask(loc[j], 0, vars, m) == 0) continue;
         if (L.klass != SMC_K_EXP) { can = true; continue; }
-        const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
+        const double r = location_rate(L, vars, m, memo);
         if (r > 0.0) can = true;
     }
     return (__ballot(can) & seg_mask) == 0;

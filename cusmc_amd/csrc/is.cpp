@@ -177,7 +177,8 @@ struct smc_is {
     DevBuf<uint16_t> th1, th2, Gb, dh2b, dh1b, xin;
     DevBuf<float> tth, tthg, dh2, dh1, work, xf;
     DevBuf<unsigned int> counters;  // [0] active rows, [1] record slots used, [2] compaction count
-    unsigned int *h_counters = nullptr;
+    unsigned int *h_counters = nullptr;  // [0..1] copies of counters[0..1]; [2..3] active counts of alternate macro-steps
+    hipEvent_t step_done[2] = {nullptr, nullptr};
     int adam_t = 0;
     bool zero_output = true;  // output layer all zero: theta == 0 exactly
     bool trained = false;
@@ -185,6 +186,7 @@ struct smc_is {
     ~smc_is() {
         if (m) hipSetDevice(m->device);
         if (h_counters) hipHostFree(h_counters);
+        for (hipEvent_t e : step_done) if (e) hipEventDestroy(e);
     }
 
     hipStream_t stream() const { return m->stream; }
@@ -243,6 +245,8 @@ struct smc_is {
         P.tables = tab ? (decltype(P.tables))m->arena : nullptr;
         P.table_bytes = tab ? m->table_bytes : 0;
         P.wave_lds = wave_lds;
+        P.theta_cap = Gpad;
+        P.theta_len = 0;
         P.var_scale = var_scale.p;
         P.weight = weight.p;
         P.active_count = counters.p;
@@ -363,17 +367,29 @@ struct smc_is {
         P.early_stop = b.early_stop ? 1 : 0;
         P.trace = b.trace ? 1 : 0;
         P.stop_at_transition = b.lockstep ? 1 : 0;
-        if (b.tilt == Tilt::FIXED) { P.theta = theta_fixed.p; P.theta_stride = 0; P.tilt_index = tix_loc.p; }
+        if (b.tilt == Tilt::FIXED) {
+            P.theta = theta_fixed.p;
+            P.theta_stride = 0;
+            P.theta_len = Lpad;
+            P.tilt_index = tix_loc.p;
+        }
         if (b.record) {
             P.rec_X = rX.p; P.rec_A = rA.p; P.rec_B = rB.p; P.rec_w = rw.p;
             hip_check(hipMemsetAsync(counters.p + 1, 0, sizeof(unsigned int), s), "clear record count");
         }
+        // Lock-step macro-steps.  The host never waits for the step it just
+        // launched: after launching step k it reads the active count of step
+        // k-1 (an upper bound on the rows still live after step k), so the
+        // GPU always has the next step queued.  A count of zero ends the loop
+        // (step k then found no live row); compaction sizes the new row set
+        // from the same bound.
         int launches = 0;
-        for (;;) {
+        for (int k = 0;; ++k) {
             if (feats) {
                 // the tilts from the features of the rows' states (the
                 // advance kernel then overwrites the features, in order)
                 P.theta_stride = forward(A->X.p, rows, nullptr, nullptr, thg.p, true, nullptr);
+                P.theta_len = P.theta_stride;
                 P.theta = thg.p;
                 P.tilt_index = tix_group.p;
             }
@@ -382,14 +398,18 @@ struct smc_is {
                           "record slots");
                 P.rec_pos = pos.p;
             }
-            hip_check(hipMemsetAsync(counters.p, 0, sizeof(unsigned int), s), "clear active count");
+            unsigned int *cnt = counters.p + 3 + (k & 1);
+            hip_check(hipMemsetAsync(cnt, 0, sizeof(unsigned int), s), "clear active count");
+            P.active_count = cnt;
             hip_check(smc::is_launch_advance(P, C, occ, lds, max_groups, s), "advance kernel launch");
             ++launches;
             if (!b.lockstep) break;
-            hip_check(hipMemcpyAsync(h_counters, counters.p, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, s),
+            hip_check(hipMemcpyAsync(h_counters + 2 + (k & 1), cnt, sizeof(unsigned int), hipMemcpyDeviceToHost, s),
                       "copy active count");
-            hip_check(hipStreamSynchronize(s), "advance");
-            const int active = (int)h_counters[0];
+            hip_check(hipEventRecord(step_done[k & 1], s), "event");
+            if (k == 0) continue;
+            hip_check(hipEventSynchronize(step_done[(k - 1) & 1]), "advance");
+            const int active = (int)h_counters[2 + ((k - 1) & 1)];
             if (active == 0) break;
             if (active * 2 <= rows && rows > 1024) {
                 // compaction: results of the finished rows out, live rows
@@ -535,20 +555,6 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         if (s->C > 16) throw std::runtime_error("limit: more than 1024 components");
         s->R = 64 / s->S;
         s->stride = (img.n_vars > 0 ? img.n_vars : 1) | 1;
-        s->wave_lds = (uint32_t)rup((size_t)s->R * s->stride * sizeof(double), 16);
-        if (s->wave_lds > 60000) throw std::runtime_error("limit: variables exceed the LDS budget");
-        // hot tables staged in LDS when the model's tuned launch does so (or
-        // SMC_IS_TABLES=1/0 says) and four waves' variables fit beside them
-        {
-            const char *tv = std::getenv("SMC_IS_TABLES");
-            bool want = m->has_tuned && m->tuned.tables == 1;
-            if (tv) want = std::string(tv) == "1";
-            const size_t need = (size_t)m->table_bytes + 4 * (size_t)s->wave_lds;
-            s->tab = want && m->table_bytes > 0 && need <= 160 * 1024;
-            s->lds = s->tab ? need : s->wave_lds;
-            const int per_cu = s->tab ? std::max(1, std::min(s->occ == 2 ? 2 : 1, (int)((160 * 1024) / need))) : 0;
-            s->max_groups = per_cu * m->cu_count;
-        }
 
         s->n_locs = (int)img.locs.size();
         s->n_vars = img.n_vars;
@@ -599,6 +605,25 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         }
         s->Gpad = (int)rup((size_t)std::max(s->n_groups, 1), 64);
         s->G16 = (int)rup((size_t)std::max(s->n_groups, 1), 16);
+        // advance kernel LDS: per wave the variables of its 64/S rows and
+        // their tilt tables (theta and 2^theta, up to Gpad entries), after a
+        // shared part (tilt index and a shared 2^theta vector, Lpad each)
+        s->wave_lds = (uint32_t)rup((size_t)s->R * s->stride * sizeof(double) + (size_t)s->R * s->Gpad * 12, 16);
+        if (s->wave_lds > 60000) throw std::runtime_error("limit: variables exceed the LDS budget");
+        {
+            const size_t shared = (size_t)s->Lpad * 12;
+            // hot tables staged in LDS when the model's tuned launch does so
+            // (or SMC_IS_TABLES=1/0 says) and four waves fit beside them
+            const char *tv = std::getenv("SMC_IS_TABLES");
+            bool want = m->has_tuned && m->tuned.tables == 1;
+            if (tv) want = std::string(tv) == "1";
+            const size_t need = (size_t)m->table_bytes + shared + 4 * (size_t)s->wave_lds;
+            s->tab = want && m->table_bytes > 0 && need <= 160 * 1024;
+            s->lds = s->tab ? need : shared + s->wave_lds;
+            if (s->lds > 160 * 1024) throw std::runtime_error("limit: the advance kernel's LDS exceeds the budget");
+            const int per_cu = s->tab ? std::max(1, std::min(s->occ == 2 ? 2 : 1, (int)((160 * 1024) / need))) : 0;
+            s->max_groups = per_cu * m->cu_count;
+        }
         // the fused forward pass when its weights fit in LDS;
         // SMC_IS_FORWARD=gemm forces the three-GEMM path (the two agree bit
         // for bit: same products in the same order)
@@ -668,8 +693,12 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         s->var_scale.ensure(scale.size(), "hipMalloc scales");
         s->theta_fixed.ensure(L, "hipMalloc tilts");
         s->feat0.ensure(fp, "hipMalloc features");
-        s->counters.ensure(3, "hipMalloc counters");
-        hip_check(hipHostMalloc((void **)&s->h_counters, 2 * sizeof(unsigned int), hipHostMallocDefault), "hipHostMalloc");
+        // counters: [0] active rows, [1] recorded pairs, [2] compaction slots,
+        // [3..4] active rows after alternate macro-steps
+        s->counters.ensure(5, "hipMalloc counters");
+        hip_check(hipHostMalloc((void **)&s->h_counters, 4 * sizeof(unsigned int), hipHostMallocDefault), "hipHostMalloc");
+        for (hipEvent_t &e : s->step_done)
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipMemcpyAsync(s->tiltable.p, s->h_tiltable.data(), L, hipMemcpyHostToDevice, st), "upload");
         hip_check(hipMemcpyAsync(s->var_scale.p, scale.data(), scale.size() * 4, hipMemcpyHostToDevice, st), "upload");
         hip_check(hipMemcpyAsync(s->feat0.p, f0.data(), fp * 2, hipMemcpyHostToDevice, st), "upload");

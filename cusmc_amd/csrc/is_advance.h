@@ -80,8 +80,19 @@ __device__ __forceinline__ bool settled(const LaneQuery &q, const IsParams &P, i
 
 // ------------------------------------------------------------ advance
 
+// Tilt lookups of the workgroup and its waves (all in LDS): the tilt index
+// of every location, and 2^theta precomputed once per theta entry — for the
+// whole launch when every run shares one theta vector, else per row.
+struct TiltLds {
+    const int32_t *tix;   // [lpad]
+    const double *ex;     // [theta_len]: 2^theta, shared theta only
+    double *row_ex;       // the wave's rows: [64/S][theta_cap] 2^theta
+    float *row_th;        // [64/S][theta_cap] theta
+};
+
 template <int C, class MT>
-__device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, double *wlds, int group) {
+__device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, double *wlds, const TiltLds &tl,
+                                                int group) {
     const int lane = threadIdx.x & 63;
     const int S = P.lanes;
     const int sl = lane & (S - 1);
@@ -122,7 +133,27 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
         wins[j] = 0;
     }
     wave_sync();
-    const float *theta = P.theta ? P.theta + (size_t)row * P.theta_stride : nullptr;
+    // this row's tilts: theta and 2^theta by theta column
+    const float *theta = nullptr;
+    const double *tex = nullptr;
+    if (P.theta) {
+        if (P.theta_stride == 0) {
+            theta = P.theta;
+            tex = tl.ex;
+        } else {
+            float *th = tl.row_th + (size_t)(lane / S) * P.theta_cap;
+            double *ex = tl.row_ex + (size_t)(lane / S) * P.theta_cap;
+            const float *src = P.theta + (size_t)row * P.theta_stride;
+            for (int i = sl; i < P.theta_len; i += S) {
+                const float v = src[i];
+                th[i] = v;
+                ex[i] = smc_exp2((double)v);
+            }
+            wave_sync();
+            theta = th;
+            tex = ex;
+        }
+    }
     const int rec = P.rec_pos ? P.rec_pos[row] : -1;
     if (rec >= 0) {
         write_features<C>(P, P.rec_X + (size_t)rec * P.fpad, loc, vars, t, sl, S, n);
@@ -177,6 +208,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     if (absorbed<C>(loc, vars, m, sl, n, seg_mask)) { end = true; break; }
                 }
                 double leaf[C], leaft[C], Mb[C];
+                RateMemo memo;
                 int zero_j = C;
                 uint32_t bmask = 0, deadmask = 0;
 #pragma unroll
@@ -203,12 +235,12 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                             } else if (L.flags & SMC_LF_DEAD) {
                                 deadmask |= 1u << j;
                             } else {
-                                const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
+                                const double r = location_rate(L, vars, m, memo);
                                 leaf[j] = r > 0.0 ? r : 0.0;
                                 leaft[j] = leaf[j];
                                 if (theta && leaf[j] > 0.0) {
-                                    const int ti = P.tilt_index[loc[j]];
-                                    if (ti >= 0) leaft[j] = leaf[j] * smc_exp2((double)theta[ti]);
+                                    const int ti = tl.tix[loc[j]];
+                                    if (ti >= 0) leaft[j] = leaf[j] * tex[ti];
                                 }
                             }
                         }
@@ -222,12 +254,12 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                         const int j = __ffs(b) - 1;
                         const int lj = slot_get<C>(loc, j);
                         const DevLoc L = load_rec(m.locs + lj);
-                        const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
+                        const double r = location_rate(L, vars, m, memo);
                         const double lf = r > 0.0 ? r : 0.0;
                         double lt = lf;
                         if (theta && lf > 0.0) {
-                            const int ti = P.tilt_index[lj];
-                            if (ti >= 0) lt = lf * smc_exp2((double)theta[ti]);
+                            const int ti = tl.tix[lj];
+                            if (ti >= 0) lt = lf * tex[ti];
                         }
                         slot_set<C>(leaf, j, lf);
                         slot_set<C>(leaft, j, lt);
@@ -327,7 +359,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                             double tw = 0.0;
                             if (lane == wl_lane) {
                                 const int lw = slot_get<C>(loc, wj);
-                                const int ti = P.tilt_index[lw];
+                                const int ti = tl.tix[lw];
                                 if (ti >= 0) tw = (double)theta[ti];
                             }
                             th_w = __shfl(tw, wl_lane);
@@ -478,19 +510,33 @@ template <int C, bool TAB>
 __device__ __forceinline__ void is_advance_body(const IsParams &P) {
     extern __shared__ __attribute__((aligned(16))) double lds[];
     const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
-    double *wlds = lds + (TAB ? P.table_bytes / 8 : 0) + (size_t)wv * (P.wave_lds / 8);
+    // LDS: [tables: TAB only][tilt index: lpad int32][2^theta: lpad doubles]
+    //      then per wave [variables][rows' 2^theta][rows' theta]
+    double *shared = lds + (TAB ? P.table_bytes / 8 : 0);
+    int32_t *tix = (int32_t *)shared;
+    double *ex = shared + P.lpad / 2;
+    double *wlds = shared + (size_t)P.lpad * 3 / 2 + (size_t)wv * (P.wave_lds / 8);
     const int R = 64 / P.lanes;
+    TiltLds tl;
+    tl.tix = tix;
+    tl.ex = ex;
+    tl.row_ex = wlds + (size_t)R * P.var_stride;
+    tl.row_th = (float *)(tl.row_ex + (size_t)R * P.theta_cap);
+    for (int i = threadIdx.x; i < P.lpad; i += blockDim.x) tix[i] = P.tilt_index[i];
+    if (P.theta && P.theta_stride == 0)
+        for (int i = threadIdx.x; i < P.theta_len; i += blockDim.x) ex[i] = smc_exp2((double)P.theta[i]);
     const int groups = (P.n_rows + R - 1) / R;
     if constexpr (TAB) {
-        stage_tables(P.tables, P.table_bytes, lds);
+        stage_tables(P.tables, P.table_bytes, lds);  // ends with the workgroup barrier
         const DevModelT<SmcLdsTables> m = lds_model(P.m, P.tables, lds);
         for (int g = blockIdx.x * W + wv; g < groups; g += gridDim.x * W) {
-            is_advance_rows<C>(P, m, wlds, g);
+            is_advance_rows<C>(P, m, wlds, tl, g);
             wave_sync();
         }
     } else {
+        __syncthreads();
         for (int g = blockIdx.x * W + wv; g < groups; g += gridDim.x * W) {
-            is_advance_rows<C>(P, P.m, wlds, g);
+            is_advance_rows<C>(P, P.m, wlds, tl, g);
             wave_sync();
         }
     }

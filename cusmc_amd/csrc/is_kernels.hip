@@ -344,6 +344,8 @@ __global__ void mlp_pack_kernel(const float *w, size_t o_w1, size_t o_b1, size_t
     }
 }
 
+constexpr int kMlpMaxK = 32;  // layer-1 k-steps held in registers: fpad <= 512
+
 template <int H>
 __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpFwdArgs a) {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
@@ -374,16 +376,23 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpFwdArgs a) {
         f32x4 acc[NT];
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // layer 1
+        // layer 1: the tile's A fragments are loaded first, all at once (the
+        // loads overlap instead of each k-step waiting on memory)
         const uint16_t *xrow = a.X + (size_t)(r0 + ar) * a.fpad + kq;
-        for (int k0 = 0; k0 < a.fpad; k0 += 16) {
-            const bf16x4 fa = *(const bf16x4 *)(xrow + k0);
+        const int KT = a.fpad / 16;
+        bf16x4 xa[kMlpMaxK];
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                const bf16x4 fb = *(const bf16x4 *)(w1t + (size_t)(nt * 16 + ar) * KP1 + k0 + kq);
-                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fb, acc[nt], 0, 0, 0);
+        for (int kt = 0; kt < kMlpMaxK; ++kt)
+            if (kt < KT) xa[kt] = *(const bf16x4 *)(xrow + kt * 16);
+#pragma unroll
+        for (int kt = 0; kt < kMlpMaxK; ++kt)
+            if (kt < KT) {
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const bf16x4 fb = *(const bf16x4 *)(w1t + (size_t)(nt * 16 + ar) * KP1 + kt * 16 + kq);
+                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xa[kt], fb, acc[nt], 0, 0, 0);
+                }
             }
-        }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
             const int col = nt * 16 + cc;
@@ -504,7 +513,8 @@ static int max_lds_per_block() {
 }
 
 bool mlp_forward_supported(int fpad, int H, int g16, float clamp_max) {
-    if (!(H == 64 || H == 128 || H == 256) || fpad <= 0 || fpad % 16 || g16 <= 0 || g16 % 16 || g16 > 64 ||
+    if (!(H == 64 || H == 128 || H == 256) || fpad <= 0 || fpad % 16 || fpad > 16 * kMlpMaxK || g16 <= 0 ||
+        g16 % 16 || g16 > 64 ||
         !(clamp_max > 0.f))
         return false;
     return mlp_lds_bytes(fpad, H, g16) <= (size_t)max_lds_per_block();

@@ -51,7 +51,9 @@ struct IsParams {
     const int32_t *tilt_index;  // [lpad]: the location's column in a theta row, -1 if untiltable
     const SMC_GLOBAL uint8_t *tables;  // hot tables staged in LDS when table_bytes > 0: arena start
     uint32_t table_bytes;       // bytes staged (multiple of 16)
-    uint32_t wave_lds;          // LDS bytes of one wave after the tables (multiple of 16)
+    uint32_t wave_lds;          // LDS bytes of one wave after the shared part (multiple of 16)
+    int32_t theta_len;          // entries of a theta row (<= lpad when theta_stride == 0, else <= theta_cap)
+    int32_t theta_cap;          // per-row tilt entries each wave's LDS holds room for
     uint16_t *feat;             // [rows][fpad] bf16 features of the end state, or null
     const float *var_scale;     // [n_vars]
     // recording of cross-entropy statistics (replay of elite runs)

@@ -171,6 +171,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                     double leaf[C], Mb[C];
                     int zero_j = C;
                     uint32_t bmask = 0, deadmask = 0;
+                    RateMemo memo;
 #pragma unroll
                     for (int j = 0; j < C; ++j) {
                         leaf[j] = 0.0;
@@ -196,7 +197,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                                 } else if (L.flags & SMC_LF_DEAD) {
                                     deadmask |= 1u << j;
                                 } else {
-                                    const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
+                                    const double r = location_rate(L, vars, m, memo);
                                     leaf[j] = r > 0.0 ? r : 0.0;
                                 }
                             }
@@ -212,7 +213,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                         for (uint32_t b = deadmask; b; b &= b - 1) {
                             const int j = __ffs(b) - 1;
                             const DevLoc L = load_rec(m.locs + slot_get<C>(loc, j));
-                            const double r = L.rate_const ? L.rate_value : eval_ref(L.rate, vars, m);
+                            const double r = location_rate(L, vars, m, memo);
                             slot_set<C>(leaf, j, r > 0.0 ? r : 0.0);
                         }
                     }
