@@ -42,6 +42,11 @@ WAVE_PEAK = 256 * 2.4e9             # wave64 VALU instructions per second
 # algorithmic lane-ops per race step (oracle cost model; tests pin them)
 COST_PER_STEP = {"fischer32": 334.7, "sir350": 2626.3, "race8": 881.2}
 SEED = 20261202
+# HBM bytes per run moved by the run kernel (reads + writes), from rocprofv3
+# FETCH_SIZE / WRITE_SIZE counters scaled by what they read for a 1 GiB device
+# copy, for the launch shape (lanes per run, register policy, tables in LDS)
+# they were taken on; another shape reports no traffic
+TRAFFIC_PER_RUN = {"fischer32": ((4, 2, 1), 3334.4), "sir350": ((64, 2, 0), 50212.8)}
 
 
 def _models(workload):
@@ -88,6 +93,9 @@ def bench_plain(args) -> dict:
     steps = sum(r.steps for r in results)
     kms = sum(r.kernel_ms for r in results)
     achieved = COST_PER_STEP[args.workload] * steps / (kms / 1e3)
+    used = (results[-1].lanes_per_run, policy, int(bool(tables)))
+    shape_t, per_run_t = TRAFFIC_PER_RUN.get(args.workload, (None, None))
+    traffic = per_run_t * runs if shape_t == used else None
     line = {
         "metric": "runs/sec; transitions/sec reported alongside",
         "value": n / wall,
@@ -111,7 +119,7 @@ def bench_plain(args) -> dict:
         "estimate": results[-1].estimate,
         "ch_eps": results[-1].ch_eps,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": LANE_PEAK, "unit": "lane-op/s",
-                     "frac": achieved / LANE_PEAK, "traffic": None,
+                     "frac": achieved / LANE_PEAK, "traffic": traffic,
                      "work": f"{COST_PER_STEP[args.workload]} algorithmic lane-ops per race step x "
                              f"{steps // args.steps} race steps per launch"},
     }

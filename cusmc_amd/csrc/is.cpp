@@ -373,9 +373,14 @@ struct smc_is {
             P.theta_len = Lpad;
             P.tilt_index = tix_loc.p;
         }
+        size_t lds_use = lds;
         if (b.record) {
             P.rec_X = rX.p; P.rec_A = rA.p; P.rec_B = rB.p; P.rec_w = rw.p;
             hip_check(hipMemsetAsync(counters.p + 1, 0, sizeof(unsigned int), s), "clear record count");
+            // the recording build reads the tables from global memory
+            P.tables = nullptr;
+            P.table_bytes = 0;
+            lds_use = (size_t)Lpad * 12 + wave_lds;
         }
         // Lock-step macro-steps.  The host never waits for the step it just
         // launched: after launching step k it reads the active count of step
@@ -401,7 +406,7 @@ struct smc_is {
             unsigned int *cnt = counters.p + 3 + (k & 1);
             hip_check(hipMemsetAsync(cnt, 0, sizeof(unsigned int), s), "clear active count");
             P.active_count = cnt;
-            hip_check(smc::is_launch_advance(P, C, occ, lds, max_groups, s), "advance kernel launch");
+            hip_check(smc::is_launch_advance(P, C, occ, lds_use, max_groups, s), "advance kernel launch");
             ++launches;
             if (!b.lockstep) break;
             hip_check(hipMemcpyAsync(h_counters + 2 + (k & 1), cnt, sizeof(unsigned int), hipMemcpyDeviceToHost, s),

@@ -90,7 +90,9 @@ struct TiltLds {
     float *row_th;        // [64/S][theta_cap] theta
 };
 
-template <int C, class MT>
+// REC: the build that records macro-step statistics (elite replays); the
+// sampling build leaves the recording state out of its registers.
+template <int C, bool REC, class MT>
 __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, double *wlds, const TiltLds &tl,
                                                 int group) {
     const int lane = threadIdx.x & 63;
@@ -154,7 +156,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
             tex = ex;
         }
     }
-    const int rec = P.rec_pos ? P.rec_pos[row] : -1;
+    const int rec = (REC && P.rec_pos) ? P.rec_pos[row] : -1;
     if (rec >= 0) {
         write_features<C>(P, P.rec_X + (size_t)rec * P.fpad, loc, vars, t, sl, S, n);
         if (sl == 0) P.rec_w[rec] = P.weight[st.job[row]];
@@ -506,7 +508,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
 // (blockIdx.x * waves + wave, then strided by the grid).  With TAB the
 // workgroup first stages the model's hot tables in LDS (as the run kernel).
 // LDS: [tables: TAB only][per wave: variables of its rows].
-template <int C, bool TAB>
+template <int C, bool TAB, bool REC>
 __device__ __forceinline__ void is_advance_body(const IsParams &P) {
     extern __shared__ __attribute__((aligned(16))) double lds[];
     const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
@@ -530,13 +532,13 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
         stage_tables(P.tables, P.table_bytes, lds);  // ends with the workgroup barrier
         const DevModelT<SmcLdsTables> m = lds_model(P.m, P.tables, lds);
         for (int g = blockIdx.x * W + wv; g < groups; g += gridDim.x * W) {
-            is_advance_rows<C>(P, m, wlds, tl, g);
+            is_advance_rows<C, REC>(P, m, wlds, tl, g);
             wave_sync();
         }
     } else {
         __syncthreads();
         for (int g = blockIdx.x * W + wv; g < groups; g += gridDim.x * W) {
-            is_advance_rows<C>(P, P.m, wlds, tl, g);
+            is_advance_rows<C, REC>(P, P.m, wlds, tl, g);
             wave_sync();
         }
     }
@@ -544,14 +546,14 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
 
 // Registers unrestricted, or at most 256 (two waves per SIMD); see the run
 // kernel.
-template <int C, bool TAB>
+template <int C, bool TAB, bool REC>
 __global__ __launch_bounds__(256) void is_advance_kernel(IsParams P) {
-    is_advance_body<C, TAB>(P);
+    is_advance_body<C, TAB, REC>(P);
 }
 
-template <int C, bool TAB>
+template <int C, bool TAB, bool REC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void is_advance_kernel_o2(IsParams P) {
-    is_advance_body<C, TAB>(P);
+    is_advance_body<C, TAB, REC>(P);
 }
 
 }  // namespace

@@ -15,12 +15,17 @@ namespace {
 
 hipError_t launch(const IsParams &p, int grid, int group_waves, size_t lds, hipStream_t s) {
     const dim3 g(grid), b(64 * group_waves);
+    // builds: staged tables (sampling), global tables (sampling), global
+    // tables with recording
+    if (p.rec_pos && p.table_bytes) return hipErrorInvalidValue;
 #if SMC_OCC == 2
-    if (p.table_bytes) hipLaunchKernelGGL((is_advance_kernel_o2<SMC_C, true>), g, b, lds, s, p);
-    else hipLaunchKernelGGL((is_advance_kernel_o2<SMC_C, false>), g, b, lds, s, p);
+    if (p.rec_pos) hipLaunchKernelGGL((is_advance_kernel_o2<SMC_C, false, true>), g, b, lds, s, p);
+    else if (p.table_bytes) hipLaunchKernelGGL((is_advance_kernel_o2<SMC_C, true, false>), g, b, lds, s, p);
+    else hipLaunchKernelGGL((is_advance_kernel_o2<SMC_C, false, false>), g, b, lds, s, p);
 #else
-    if (p.table_bytes) hipLaunchKernelGGL((is_advance_kernel<SMC_C, true>), g, b, lds, s, p);
-    else hipLaunchKernelGGL((is_advance_kernel<SMC_C, false>), g, b, lds, s, p);
+    if (p.rec_pos) hipLaunchKernelGGL((is_advance_kernel<SMC_C, false, true>), g, b, lds, s, p);
+    else if (p.table_bytes) hipLaunchKernelGGL((is_advance_kernel<SMC_C, true, false>), g, b, lds, s, p);
+    else hipLaunchKernelGGL((is_advance_kernel<SMC_C, false, false>), g, b, lds, s, p);
 #endif
     return hipGetLastError();
 }

@@ -166,7 +166,13 @@ SMC_HD double smc_exp2(double y) {
 /* Power with a reproducible result: binary exponentiation for integer
  * exponents |b| <= 1024, otherwise 2^(b log2 a) for a > 0, NaN for a < 0
  * with a non-integer exponent, and the IEEE special cases for a == 0. */
+#if defined(__HIP_DEVICE_COMPILE__)
+/* Out of line on the GPU: models rarely raise to powers, and inlined into
+ * every expression site its temporaries set the kernels' register peak. */
+__attribute__((noinline)) __device__ double smc_pow(double a, double b) {
+#else
 SMC_HD double smc_pow(double a, double b) {
+#endif
     if (b == floor(b) && fabs(b) <= 1024.0) {
         long n = (long)(b < 0 ? -b : b);
         double r = 1.0, x = a;
