@@ -138,12 +138,27 @@ __device__ __forceinline__ const SMC_LDS T *in_lds(const SMC_GLOBAL T *p, const 
     return (const SMC_LDS T *)(copy + ((const SMC_GLOBAL uint8_t *)p - base));
 }
 
+// Workgroup-wide copy of n 16-byte words into LDS, eight loads in flight
+// per thread before their stores (a plain loop waits on every load).
+__device__ __forceinline__ void copy_to_lds(const SMC_GLOBAL uint4 *src, SMC_LDS uint4 *dst, uint32_t n) {
+    const uint32_t step = blockDim.x;
+    for (uint32_t i = threadIdx.x; i < n; i += 8 * step) {
+        uint4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i + u * step < n) v[u] = src[i + u * step];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i + u * step < n) dst[i + u * step] = v[u];
+    }
+}
+
 // Copy the arena prefix [0, bytes) into the start of the workgroup's LDS
 // (every thread of the workgroup takes part; ends with a barrier).
 __device__ __forceinline__ void stage_tables(const SMC_GLOBAL uint8_t *src, uint32_t bytes, double *lds) {
     const SMC_GLOBAL uint4 *s4 = (const SMC_GLOBAL uint4 *)src;
     SMC_LDS uint4 *d4 = (SMC_LDS uint4 *)(SMC_LDS double *)lds;
-    for (uint32_t i = threadIdx.x; i < bytes / 16; i += blockDim.x) d4[i] = s4[i];
+    copy_to_lds(s4, d4, bytes / 16);
     __syncthreads();
 }
 

@@ -360,12 +360,8 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpFwdArgs a) {
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
     uint16_t *hb = (uint16_t *)(b3s + g16) + (size_t)wave * 16 * KP;
 
-    {
-        const uint4 *src = (const uint4 *)a.image;
-        uint4 *dst = (uint4 *)smem;
-        const int n16 = (int)(mlp_image_size(a.fpad, H, g16) / 16);
-        for (int i = tid; i < n16; i += 256) dst[i] = src[i];
-    }
+    copy_to_lds((const SMC_GLOBAL uint4 *)a.image, (SMC_LDS uint4 *)(SMC_LDS uint16_t *)smem,
+                (uint32_t)(mlp_image_size(a.fpad, H, g16) / 16));
     __syncthreads();
 
     const int ar = l & 15, kq = 4 * (l >> 4);  // A/B fragments: row (column) and first k
