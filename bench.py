@@ -46,7 +46,7 @@ SEED = 20261202
 # FETCH_SIZE / WRITE_SIZE counters scaled by what they read for a 1 GiB device
 # copy, for the launch shape (lanes per run, register policy, tables in LDS)
 # they were taken on; another shape reports no traffic
-TRAFFIC_PER_RUN = {"fischer32": ((4, 2, 1), 3334.4), "sir350": ((64, 2, 0), 50212.8)}
+TRAFFIC_PER_RUN = {"fischer32": ((4, 2, 1), 85.4), "sir350": ((64, 2, 0), 213.1)}
 
 
 def _models(workload):
