@@ -556,6 +556,12 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
             s->S = m->tuned.lanes_per_run;
             s->occ = m->tuned.occupancy == 2 ? 2 : 0;
         }
+        // SMC_IS_LANES=<power of two> overrides the lanes per run (diagnosis:
+        // the sampler's best packing may differ from the plain simulator's)
+        if (const char *lv = std::getenv("SMC_IS_LANES")) {
+            const int L = std::atoi(lv);
+            if (L >= 1 && L <= 64 && (L & (L - 1)) == 0 && L <= P && P / L <= 16) s->S = L;
+        }
         s->C = P / s->S;
         if (s->C > 16) throw std::runtime_error("limit: more than 1024 components");
         s->R = 64 / s->S;

@@ -178,6 +178,8 @@ __global__ __launch_bounds__(64) void colsum_kernel(const float *X, int rows, in
     if (col >= N) return;
     const int r0 = blockIdx.y * rows_per, r1 = min(rows, r0 + rows_per);
     float s = 0.f;
+    // unrolled so the loads overlap; the sum stays in row order
+#pragma unroll 8
     for (int r = r0; r < r1; ++r) s += X[(size_t)r * ld + col];
     part[(size_t)blockIdx.y * N + col] = s;
 }
@@ -187,6 +189,7 @@ __global__ void fold_splits_kernel(const float *part, int splits, size_t n, floa
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float s = 0.f;
+#pragma unroll 8
     for (int z = 0; z < splits; ++z) s += part[(size_t)z * n + i];
     dst[i] = accumulate ? dst[i] + s : s;
 }
@@ -207,8 +210,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-    __shared__ uint16_t As[64][16 + 4];
-    __shared__ uint16_t Bs[16][64 + 4];
+    __shared__ __attribute__((aligned(16))) uint16_t As[64][16 + 4];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[16][64 + 4];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, l = tid & 63;
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
@@ -221,33 +224,45 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     const int kz = (int)blockIdx.z * g.kchunk;
     const int kend = min(g.K, kz + g.kchunk);
     float *const Cout = g.C ? g.C + (size_t)blockIdx.z * g.zstride : nullptr;
+    // A (64 x 16) and B (16 x 64) tiles, 1024 elements each: every thread
+    // moves four consecutive elements of each with one 8-byte load; the
+    // next tile's loads are issued before the current tile's MFMAs
+    const int ta_k = tid >> 4, ta_i = 4 * (tid & 15);  // A stored [K][M]
+    const int na_i = tid >> 2, na_k = 4 * (tid & 3);   // A stored [M][K]
+    const int tb_j = tid >> 2, tb_k = 4 * (tid & 3);   // B stored [N][K]
+    const int nb_k = tid >> 4, nb_j = 4 * (tid & 15);  // B stored [K][N]
+    auto load_a = [&](int k0) -> uint2 {
+        return TA ? *(const uint2 *)(g.A + (size_t)(k0 + ta_k) * g.lda + (m0 + ta_i))
+                  : *(const uint2 *)(g.A + (size_t)(m0 + na_i) * g.lda + (k0 + na_k));
+    };
+    auto load_b = [&](int k0) -> uint2 {
+        return TB ? *(const uint2 *)(g.B + (size_t)(n0 + tb_j) * g.ldb + (k0 + tb_k))
+                  : *(const uint2 *)(g.B + (size_t)(k0 + nb_k) * g.ldb + (n0 + nb_j));
+    };
+    uint2 ra = kz < kend ? load_a(kz) : uint2{0u, 0u};
+    uint2 rb = kz < kend ? load_b(kz) : uint2{0u, 0u};
     for (int k0 = kz; k0 < kend; k0 += 16) {
-        // stage A (64 x 16) and B (16 x 64): 1024 elements each, 4 per thread
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int idx = tid * 4 + e;
-            if (TA) {
-                const int i = idx & 63, k = idx >> 6;       // coalesced along M
-                As[i][k] = g.A[(size_t)(k0 + k) * g.lda + (m0 + i)];
-            } else {
-                const int k = idx & 15, i = idx >> 4;       // coalesced along K
-                As[i][k] = g.A[(size_t)(m0 + i) * g.lda + (k0 + k)];
-            }
-            if (TB) {
-                const int k = idx & 15, j = idx >> 4;       // coalesced along K
-                Bs[k][j] = g.B[(size_t)(n0 + j) * g.ldb + (k0 + k)];
-            } else {
-                const int j = idx & 63, k = idx >> 6;       // coalesced along N
-                Bs[k][j] = g.B[(size_t)(k0 + k) * g.ldb + (n0 + j)];
-            }
+        if (TA) {
+            As[ta_i + 0][ta_k] = (uint16_t)ra.x; As[ta_i + 1][ta_k] = (uint16_t)(ra.x >> 16);
+            As[ta_i + 2][ta_k] = (uint16_t)ra.y; As[ta_i + 3][ta_k] = (uint16_t)(ra.y >> 16);
+        } else {
+            *(uint2 *)&As[na_i][na_k] = ra;
+        }
+        if (TB) {
+            Bs[tb_k + 0][tb_j] = (uint16_t)rb.x; Bs[tb_k + 1][tb_j] = (uint16_t)(rb.x >> 16);
+            Bs[tb_k + 2][tb_j] = (uint16_t)rb.y; Bs[tb_k + 3][tb_j] = (uint16_t)(rb.y >> 16);
+        } else {
+            *(uint2 *)&Bs[nb_k][nb_j] = rb;
         }
         __syncthreads();
+        if (k0 + 16 < kend) {
+            ra = load_a(k0 + 16);
+            rb = load_b(k0 + 16);
+        }
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            bf16x4 fa;
             const int row = wr + a * 16 + (l & 15), kb = 4 * (l >> 4);
-            fa[0] = (short)As[row][kb]; fa[1] = (short)As[row][kb + 1];
-            fa[2] = (short)As[row][kb + 2]; fa[3] = (short)As[row][kb + 3];
+            const bf16x4 fa = *(const bf16x4 *)&As[row][kb];
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
                 bf16x4 fb;
