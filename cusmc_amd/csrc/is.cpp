@@ -377,10 +377,12 @@ struct smc_is {
         if (b.record) {
             P.rec_X = rX.p; P.rec_A = rA.p; P.rec_B = rB.p; P.rec_w = rw.p;
             hip_check(hipMemsetAsync(counters.p + 1, 0, sizeof(unsigned int), s), "clear record count");
-            // the recording build reads the tables from global memory
+            // the recording build reads the tables from global memory and
+            // keeps its per-slot statistics in LDS (16 C bytes per lane)
             P.tables = nullptr;
             P.table_bytes = 0;
-            lds_use = (size_t)Lpad * 12 + wave_lds;
+            P.wave_lds = wave_lds + (uint32_t)(16 * C * 64);
+            lds_use = (size_t)Lpad * 12 + P.wave_lds;
         }
         // Lock-step macro-steps.  The host never waits for the step it just
         // launched: after launching step k it reads the active count of step

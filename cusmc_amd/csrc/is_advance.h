@@ -88,6 +88,7 @@ struct TiltLds {
     const double *ex;     // [theta_len]: 2^theta, shared theta only
     double *row_ex;       // the wave's rows: [64/S][theta_cap] 2^theta
     float *row_th;        // [64/S][theta_cap] theta
+    double *rec;          // recording build: [C][64] rates, [C][64] int32 locations, [C][64] uint32 wins
 };
 
 // REC: the build that records macro-step statistics (elite replays); the
@@ -123,16 +124,21 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
     LaneQuery q{P.qvar, st.qmax[row], st.qmin[row]};
     uint64_t digest = st.digest[row];
     for (int v = sl; v < m.n_vars; v += S) vars[v] = st.vars[(size_t)row * m.n_vars + v];
-    int loc[C], loc0[C];
-    double r0[C];
-    uint32_t wins[C];
+    int loc[C];
+    // recording build: per slot the starting location, the rates at the
+    // macro-step's first race and the exponential wins, in LDS ([slot][lane])
+    double *r0 = tl.rec + lane;
+    int32_t *loc0 = (int32_t *)(tl.rec + 64 * C) + lane;
+    uint32_t *wins = (uint32_t *)(tl.rec + 64 * C) + 64 * C + lane;
 #pragma unroll
     for (int j = 0; j < C; ++j) {
         const int c = sl * C + j;
         loc[j] = c < n ? (int)st.loc[(size_t)row * n + c] : -1;
-        loc0[j] = loc[j];
-        r0[j] = 0.0;
-        wins[j] = 0;
+        if (REC) {
+            loc0[64 * j] = loc[j];
+            r0[64 * j] = 0.0;
+            wins[64 * j] = 0;
+        }
     }
     wave_sync();
     // this row's tilts: theta and 2^theta by theta column
@@ -268,8 +274,10 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     }
                 }
                 if (first_race) {
+                    if (REC) {
 #pragma unroll
-                    for (int j = 0; j < C; ++j) r0[j] = leaf[j];
+                        for (int j = 0; j < C; ++j) r0[64 * j] = leaf[j];
+                    }
                     first_race = false;
                 }
                 const uint64_t zb = __ballot(zero_j < C) & seg_mask;
@@ -370,10 +378,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     }
                     if (exp_won && rec >= 0) {
                         const int wj = w % C;
-                        if (lane == seg_base + w / C) {
-#pragma unroll
-                            for (int j = 0; j < C; ++j) wins[j] += (j == wj) ? 1u : 0u;
-                        }
+                        if (lane == seg_base + w / C) wins[64 * wj] += 1u;
                     }
                 }
                 if (!(d < INF)) { t = INF; end = true; break; }
@@ -490,9 +495,9 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
             if (c < n) {
                 for (int l = m.comp_first_loc[c]; l < m.comp_first_loc[c + 1]; ++l) {
                     float a = 0.f, b = 0.f;
-                    if (l == loc0[j] && P.tiltable[l]) {
-                        a = (float)wins[j];
-                        b = (float)(LN2 * dt * r0[j]);
+                    if (l == loc0[64 * j] && P.tiltable[l]) {
+                        a = (float)wins[64 * j];
+                        b = (float)(LN2 * dt * r0[64 * j]);
                     }
                     A[l] = a;
                     B[l] = b;
@@ -513,7 +518,8 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
     extern __shared__ __attribute__((aligned(16))) double lds[];
     const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
     // LDS: [tables: TAB only][tilt index: lpad int32][2^theta: lpad doubles]
-    //      then per wave [variables][rows' 2^theta][rows' theta]
+    //      then per wave [variables][rows' 2^theta][rows' theta][recording
+    //      build: 16 C bytes per lane]
     double *shared = lds + (TAB ? P.table_bytes / 8 : 0);
     int32_t *tix = (int32_t *)shared;
     double *ex = shared + P.lpad / 2;
@@ -524,6 +530,7 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
     tl.ex = ex;
     tl.row_ex = wlds + (size_t)R * P.var_stride;
     tl.row_th = (float *)(tl.row_ex + (size_t)R * P.theta_cap);
+    tl.rec = (double *)(tl.row_th + (size_t)R * P.theta_cap);
     for (int i = threadIdx.x; i < P.lpad; i += blockDim.x) tix[i] = P.tilt_index[i];
     if (P.theta && P.theta_stride == 0)
         for (int i = threadIdx.x; i < P.theta_len; i += blockDim.x) ex[i] = smc_exp2((double)P.theta[i]);
