@@ -145,8 +145,10 @@ __device__ __forceinline__ void copy_to_lds(const SMC_GLOBAL uint4 *src, SMC_LDS
     for (uint32_t i = threadIdx.x; i < n; i += 8 * step) {
         uint4 v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < 8; ++u) {
+            v[u] = uint4{0u, 0u, 0u, 0u};
             if (i + u * step < n) v[u] = src[i + u * step];
+        }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
             if (i + u * step < n) dst[i + u * step] = v[u];

@@ -320,19 +320,23 @@ __global__ void adam_kernel(float *w, const float *gsum, float *m1, float *m2, s
 
 // ------------------------------------------------------------ fused forward
 //
-// Persistent workgroups of four waves.  The workgroup stages the three
+// Persistent workgroups of eight waves.  The workgroup stages the three
 // weight matrices transposed ([n][k], rows padded by 4 bf16 against bank
 // conflicts) and the biases in LDS once; each wave then takes 16-row tiles:
 // layer 1 reads its A fragments straight from the feature rows, layers 2
 // and 3 from the wave's activation tile in LDS.  MFMA 16x16x16 bf16 with
 // fp32 accumulation, K in steps of 16 from 0 upwards, as gemm_kernel.
 
+// Waves per workgroup: two per SIMD share one copy of the weights, so one
+// wave's memory waits overlap the other's MFMAs.
+constexpr int kMlpWaves = 8;
+
 __host__ __device__ constexpr size_t mlp_image_size(int fpad, int H, int g16) {
     return 2 * ((size_t)H * (fpad + 4) + (size_t)H * (H + 4) + (size_t)g16 * (H + 4)) + 4 * (2 * (size_t)H + g16);
 }
 
 __host__ __device__ constexpr size_t mlp_lds_bytes(int fpad, int H, int g16) {
-    return mlp_image_size(fpad, H, g16) + 2 * (size_t)4 * 16 * (H + 4);
+    return mlp_image_size(fpad, H, g16) + 2 * (size_t)kMlpWaves * 16 * (H + 4);
 }
 
 // The packed image, one thread per element: bf16 part, then fp32 biases.
@@ -362,7 +366,7 @@ __global__ void mlp_pack_kernel(const float *w, size_t o_w1, size_t o_b1, size_t
 constexpr int kMlpMaxK = 32;  // layer-1 k-steps held in registers: fpad <= 512
 
 template <int H>
-__global__ __launch_bounds__(256) void mlp_forward_kernel(MlpFwdArgs a) {
+__global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpFwdArgs a) {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
     constexpr int NT = H / 16, KP = H + 4;
     const int KP1 = a.fpad + 4, g16 = a.g16;
@@ -382,7 +386,7 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(MlpFwdArgs a) {
     const int ar = l & 15, kq = 4 * (l >> 4);  // A/B fragments: row (column) and first k
     const int cr = 4 * (l >> 4), cc = l & 15;  // C fragment: rows cr..cr+3, column cc
     const int tiles = a.rows / 16;
-    for (int t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
+    for (int t = blockIdx.x * kMlpWaves + wave; t < tiles; t += gridDim.x * kMlpWaves) {
         const int r0 = t * 16;
         f32x4 acc[NT];
 #pragma unroll
@@ -554,10 +558,11 @@ hipError_t mlp_forward(const MlpFwdArgs &a, hipStream_t s) {
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
     const int tiles = a.rows / 16;
-    const int grid = std::max(1, std::min((tiles + 3) / 4, cus));
-    if (a.H == 64) hipLaunchKernelGGL(mlp_forward_kernel<64>, dim3(grid), dim3(256), lds, s, a);
-    else if (a.H == 128) hipLaunchKernelGGL(mlp_forward_kernel<128>, dim3(grid), dim3(256), lds, s, a);
-    else hipLaunchKernelGGL(mlp_forward_kernel<256>, dim3(grid), dim3(256), lds, s, a);
+    const int grid = std::max(1, std::min((tiles + kMlpWaves - 1) / kMlpWaves, cus));
+    const dim3 block(64 * kMlpWaves);
+    if (a.H == 64) hipLaunchKernelGGL(mlp_forward_kernel<64>, dim3(grid), block, lds, s, a);
+    else if (a.H == 128) hipLaunchKernelGGL(mlp_forward_kernel<128>, dim3(grid), block, lds, s, a);
+    else hipLaunchKernelGGL(mlp_forward_kernel<256>, dim3(grid), block, lds, s, a);
     return hipGetLastError();
 }
 
