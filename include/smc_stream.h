@@ -4,8 +4,8 @@
  *
  * cuSMC draws from per-(run, component) curand XORWOW states
  * (simulation.cu:57,515,529,758); those states are shared by every block
- * (smc.cu:228), so its streams are not reproducible.  This build replaces
- * them with a counter-based generator: Philox4x32-10 (Salmon, Moraes, Dror,
+ * (smc.cu:228), so its streams are not reproducible.  Here they are
+ * replaced by a counter-based generator: Philox4x32-10 (Salmon, Moraes, Dror,
  * Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11), keyed by the
  * seed and addressed by (step, purpose, component, run).  Any number of
  * draws can be made in any order on any device and a host reimplementation
