@@ -31,7 +31,36 @@ HOST_SOURCES = ["network.cpp", "uppaal.cpp", "compile.cpp"]
 HIP_SOURCES = ["capi.cpp", "is.cpp", "sim_kernel.hip", "is_kernels.hip"]
 # kernels compiled once per (component slots per lane, register policy)
 VARIANT_SOURCES = ["sim_variant.hip", "is_variant.hip"]
-VARIANTS = [(c, o) for c in (1, 2, 4, 8, 16) for o in (0, 2)]
+VARIANTS = {"sim_variant.hip": [(c, o) for c in (1, 2, 4, 8, 16) for o in (0, 2)],
+            # the advance kernel holds at most 8 slots per lane (networks of up
+            # to 512 components); 16 slots would triple the build time
+            "is_variant.hip": [(c, o) for c in (1, 2, 4, 8) for o in (0, 2)]}
+
+
+def _deps(src: Path, seen: set | None = None) -> set:
+    """The source and the local headers it includes, transitively."""
+    seen = set() if seen is None else seen
+    if src in seen or not src.exists():
+        return seen
+    seen.add(src)
+    for line in src.read_text(errors="replace").splitlines():
+        line = line.strip()
+        if line.startswith("#include \""):
+            name = line.split('"')[1]
+            for d in (src.parent, CSRC, ROOT / "include"):
+                if (d / name).exists():
+                    _deps(d / name, seen)
+                    break
+    return seen
+
+
+def _fresh(obj: Path, src: Path) -> bool:
+    """SMC_BUILD_INCREMENTAL=1: an object newer than its source and the
+    headers it includes is kept (a development shortcut; build() always
+    compiles)."""
+    if os.environ.get("SMC_BUILD_INCREMENTAL") != "1" or not obj.exists():
+        return False
+    return obj.stat().st_mtime > max(f.stat().st_mtime for f in _deps(src))
 
 
 def _run(cmd: list[str]) -> None:
@@ -54,19 +83,23 @@ def build_lib() -> Path:
     LIB.mkdir(parents=True, exist_ok=True)
     jobs, objs = [], []
     for s in VARIANT_SOURCES:  # the longest jobs first
-        for c, o in sorted(VARIANTS, key=lambda v: -v[0]):
+        for c, o in sorted(VARIANTS[s], key=lambda v: -v[0]):
             obj = _variant_obj(s, c, o)
+            objs.append(obj)
+            if _fresh(obj, CSRC / s):
+                continue
             jobs.append([HIPCC, *COMMON, *HIP_FLAGS, "-x", "hip", f"-DSMC_C={c}", f"-DSMC_OCC={o}", "-c",
                          str(CSRC / s), "-o", str(obj)])
-            objs.append(obj)
     for s in HIP_SOURCES:
         extra = ["-x", "hip"] if s.endswith(".hip") else []
-        jobs.append([HIPCC, *COMMON, *HIP_FLAGS, *extra, "-c", str(CSRC / s), "-o", str(_obj(s))])
         objs.append(_obj(s))
+        if not _fresh(_obj(s), CSRC / s):
+            jobs.append([HIPCC, *COMMON, *HIP_FLAGS, *extra, "-c", str(CSRC / s), "-o", str(_obj(s))])
     for s in HOST_SOURCES:
-        jobs.append([CXX, *COMMON, "-Wall", "-c", str(CSRC / s), "-o", str(_obj(s))])
         objs.append(_obj(s))
-    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "0")) or os.cpu_count() or 8))
+        if not _fresh(_obj(s), CSRC / s):
+            jobs.append([CXX, *COMMON, "-Wall", "-c", str(CSRC / s), "-o", str(_obj(s))])
+    workers = max(1, min(len(jobs) or 1, int(os.environ.get("MAX_JOBS", "0")) or os.cpu_count() or 8))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         list(ex.map(_run, jobs))
     out = LIB / "libcusmc.so"

@@ -306,6 +306,12 @@ ModelImage compile_model(const smc_network &n) {
         clock_memo[x] = r ? 1 : 0;
         return r;
     };
+    // Null racers leave an all-exponential race only while every rate is
+    // constant between transitions (a rate reading a clock changes at each
+    // null step of the reference's race).
+    bool clock_rate = false;
+    for (int l = 0; l < n.n_locations; ++l)
+        if (reads_clock(n.locations[l].rate)) clock_rate = true;
     for (int l = 0; l < n.n_locations; ++l) {
         const smc_location &L = n.locations[l];
         bool tf = !reads_clock(L.rate);
@@ -318,7 +324,7 @@ ModelImage compile_model(const smc_network &n) {
                 if (n.var_kind[G.var] == SMC_VAR_CLOCK || reads_clock(G.rhs)) tf = false;
             }
         }
-        bool dead = img.locs[l].klass == SMC_K_EXP;
+        bool dead = img.locs[l].klass == SMC_K_EXP && !clock_rate;
         for (int k = 0; k < L.edge_count && dead; ++k) {
             const smc_edge &E = n.edges[L.edge_first + k];
             if (E.channel < 0) continue;

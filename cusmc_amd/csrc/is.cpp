@@ -552,9 +552,10 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         // lanes per run: the model's tuned packing (smc_autotune) when there
         // is one, else the widest
         const int P = img.n_leaves;
+        if (P > 512) throw std::runtime_error("limit: importance sampling takes networks of at most 512 components");
         s->S = P < 64 ? P : 64;
         if (m->has_tuned && m->tuned.lanes_per_run > 0 && m->tuned.lanes_per_run <= s->S &&
-            P / m->tuned.lanes_per_run <= 16) {
+            P / m->tuned.lanes_per_run <= 8) {
             s->S = m->tuned.lanes_per_run;
             s->occ = m->tuned.occupancy == 2 ? 2 : 0;
         }
@@ -562,10 +563,10 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         // the sampler's best packing may differ from the plain simulator's)
         if (const char *lv = std::getenv("SMC_IS_LANES")) {
             const int L = std::atoi(lv);
-            if (L >= 1 && L <= 64 && (L & (L - 1)) == 0 && L <= P && P / L <= 16) s->S = L;
+            if (L >= 1 && L <= 64 && (L & (L - 1)) == 0 && L <= P && P / L <= 8) s->S = L;
         }
         s->C = P / s->S;
-        if (s->C > 16) throw std::runtime_error("limit: more than 1024 components");
+        if (s->C > 8) throw std::runtime_error("internal: more than 8 slots per lane");
         s->R = 64 / s->S;
         s->stride = (img.n_vars > 0 ? img.n_vars : 1) | 1;
 

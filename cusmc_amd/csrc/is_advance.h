@@ -211,7 +211,10 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
             if (phase == 0 && chan <= 0) continue;
             bool exp_won = false;
             if (phase == 1) {
-                if (changed) {
+                // a goal entered by receivers: this step is raced in full
+                // and decides the hit (as in the run kernel)
+                const bool full = P.reach && goal;
+                if (changed && !full) {
                     changed = false;
                     if (absorbed<C>(loc, vars, m, sl, n, seg_mask)) { end = true; break; }
                 }
@@ -255,9 +258,9 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     }
                 }
                 // components that cannot fire race only beside a uniform
-                // racer (as in the run kernel)
+                // racer or in a full race (as in the run kernel)
                 const bool uniform = (__ballot(bmask != 0) & seg_mask) != 0;
-                if (uniform) {
+                if (uniform || full) {
                     for (uint32_t b = deadmask; b; b &= b - 1) {
                         const int j = __ffs(b) - 1;
                         const int lj = slot_get<C>(loc, j);
@@ -352,7 +355,8 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     else { d = dexp; w = wexp; exp_won = true; }
                     // the step that crosses the bound: the left-out null
                     // racers' first event after it (base rates, no tilt)
-                    if (!uniform && d < INF && !(t + d < P.bound) && (__ballot(deadmask != 0) & seg_mask)) {
+                    if (!uniform && !full && d < INF && !(t + d < P.bound) &&
+                        (__ballot(deadmask != 0) & seg_mask)) {
                         const double Rd = dead_total<C>(deadmask, loc, vars, m, S);
                         if (Rd > 0.0) {
                             const smc_u32x4 rw = smc_draw(P.seed, run, step, SMC_P_DEAD, 0);

@@ -499,7 +499,6 @@ hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, 
     case 2: f = o2 ? is_advance_c2_o2() : is_advance_c2_o0(); break;
     case 4: f = o2 ? is_advance_c4_o2() : is_advance_c4_o0(); break;
     case 8: f = o2 ? is_advance_c8_o2() : is_advance_c8_o0(); break;
-    case 16: f = o2 ? is_advance_c16_o2() : is_advance_c16_o0(); break;
     default: return hipErrorInvalidValue;
     }
     return f(p, grid, W, lds, s);

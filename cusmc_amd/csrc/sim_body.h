@@ -164,7 +164,12 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
             for (int phase = 0; phase < 2; ++phase) {
                 if (phase == 0 && chan <= 0) continue;
                 if (phase == 1) {
-                    if (changed) {
+                    // a goal entered by this iteration's receivers counts
+                    // only if this step, raced by every component, keeps
+                    // t < bound (simulation.cu:409-415, 835-849): no
+                    // absorption exit and no left-out null racers then
+                    const bool full = p.reach && goal;
+                    if (changed && !full) {
                         changed = false;
                         if (absorbed<C>(loc, vars, m, sl, n, seg_mask)) { end = true; break; }
                     }
@@ -207,9 +212,10 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                     // steps.  With every racer exponential, leaving it out of
                     // the race keeps the law of everything else (memoryless
                     // races); a uniform racer is redrawn at every step, so then
-                    // it races as in the reference.
+                    // it races as in the reference, as it does when the step
+                    // decides the outcome (full).
                     const bool uniform = (__ballot(bmask != 0) & seg_mask) != 0;
-                    if (uniform) {
+                    if (uniform || full) {
                         for (uint32_t b = deadmask; b; b &= b - 1) {
                             const int j = __ffs(b) - 1;
                             const DevLoc L = load_rec(m.locs + slot_get<C>(loc, j));
@@ -302,7 +308,8 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                         else { d = dexp; w = wexp; }
                         // the step that crosses the bound: the left-out null
                         // racers' first event after the bound may come first
-                        if (!uniform && d < INF && !(t + d < p.bound) && (__ballot(deadmask != 0) & seg_mask)) {
+                        if (!uniform && !full && d < INF && !(t + d < p.bound) &&
+                            (__ballot(deadmask != 0) & seg_mask)) {
                             const double Rd = dead_total<C>(deadmask, loc, vars, m, S);
                             if (Rd > 0.0) {
                                 const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_DEAD, 0);

@@ -146,6 +146,11 @@ def eval_expr(net_ptr, e: int, vars_) -> float:
     return lib().oracle_eval_expr(net_ptr, int(e), v.ctypes.data_as(C.POINTER(C.c_double)))
 
 
+def pow(a: float, b: float) -> float:
+    """The simulator's own FP64 pow (include/smc_stream.h smc_pow)."""
+    return lib().oracle_pow(float(a), float(b))
+
+
 def tree_pick(rates, v: float):
     r = np.ascontiguousarray(rates, dtype=np.float64)
     ch = C.c_int()

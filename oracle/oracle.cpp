@@ -229,6 +229,14 @@ public:
             }
             dead_[l] = dead ? 1 : 0;
         }
+        // Leaving null racers out of an all-exponential race keeps the law
+        // only while every rate is constant between transitions: a rate that
+        // reads a clock is re-evaluated at each of cuSMC's null steps
+        // (simulation.cu:522-530), so then every component races.
+        bool clock_rate = false;
+        for (int l = 0; l < n.n_locations; ++l)
+            if (n.locations[l].rate >= 0 && reads_clock(n.locations[l].rate)) clock_rate = true;
+        if (clock_rate) std::fill(dead_.begin(), dead_.end(), (uint8_t)0);
     }
 
     // events: optional log of (step, component, edge) per fired edge
@@ -325,9 +333,17 @@ public:
                 base_rates_ = &r0;
             }
             new_macro = false;
+            // A goal entered by a broadcast receiver (this iteration's
+            // receivers; a goal entered by a winner has ended the run at the
+            // loop top) counts only if this iteration's step, raced by every
+            // component, keeps t < bound (simulation.cu:409-415 fire the
+            // receivers inside the iteration, :835-849 test the goal at the
+            // next loop top).  That step is raced in full: no absorption
+            // exit and no left-out null racers.
+            const bool goal_pending = reach && goal;
             // absorption (canonical sampling; cuSMC keeps racing null
             // steps to the bound, with the same outcome)
-            if (mode_ == ORACLE_CANONICAL && changed) {
+            if (mode_ == ORACLE_CANONICAL && changed && !goal_pending) {
                 changed = false;
                 if (absorbed()) break;
             }
@@ -336,7 +352,7 @@ public:
             int w;
             bool exp_won = false;
             if (mode_ == ORACLE_LITERAL) race_literal(run, step, &d, &w);
-            else race_canonical(run, step, t, &d, &w, &exp_won);
+            else race_canonical(run, step, t, goal_pending, &d, &w, &exp_won);
             base_rates_ = nullptr;
             if (record && exp_won) ++wins[w];
             if (!(d < HUGE_VAL)) { t = HUGE_VAL; break; }
@@ -515,8 +531,10 @@ private:
         return L.rate >= 0 ? eval(n_, L.rate, vars_.data()) : 1.0;
     }
 
-    // t: the time before the step (for the crossing of the bound)
-    void race_canonical(uint64_t run, uint32_t step, double t, double *d_out, int *w_out, bool *exp_won) {
+    // t: the time before the step (for the crossing of the bound); full:
+    // every component races, the null racers included
+    void race_canonical(uint64_t run, uint32_t step, double t, bool full, double *d_out, int *w_out,
+                        bool *exp_won) {
         const int N = n_.n_components;
         int zero = -1, wu = -1;
         double du = HUGE_VAL;
@@ -542,8 +560,9 @@ private:
         // A component that cannot fire only makes null steps: with every
         // racer exponential it leaves the race without changing the law of
         // the rest (memoryless races); beside a uniform racer, which is
-        // redrawn at every step, it races as in cuSMC.
-        if (uniform)
+        // redrawn at every step, it races as in cuSMC, and so it does when
+        // the step itself decides the outcome (full).
+        if (uniform || full)
             for (int c = 0; c < N; ++c)
                 if (n_.locations[loc_[c]].type <= SMC_LOC_GOAL && dead_[loc_[c]] && !clock_bounded_[loc_[c]]) {
                     const double r = rate_of(c);
@@ -585,7 +604,7 @@ private:
         // the step that crosses the bound: the left-out null racers' first
         // event after the bound (base rates) may come first; then the step
         // is a null step (w = -1)
-        if (!uniform && *d_out < HUGE_VAL && !(t + *d_out < q_.time_bound)) {
+        if (!uniform && !full && *d_out < HUGE_VAL && !(t + *d_out < q_.time_bound)) {
             bool any = false;
             dleaf_.assign(N, 0.0);
             for (int c = 0; c < N; ++c)

@@ -98,3 +98,73 @@ def test_batch_with_autotune(sims):
     for (s, q), r in zip(sims, res):
         e = s.run(q, seed=11, n_runs=2000)
         assert r.hits == e.hits and r.steps == e.steps
+
+
+def test_c4_batch_bit_exact_vs_oracle():
+    """The batched workload at its real shape: 256 SIR networks (N
+    log-uniform in [16, 512], beta and gamma from the seed, query
+    Pr[<=100](<> I > N/5)) and 256 Fischer networks (N in {4..64},
+    Pr[<=300](<> P0.cs)) in one batch round; each instance's hits, steps,
+    transitions and value sums over its runs equal the canonical oracle's on
+    the same runs (integer counts: any difference in any trajectory shows)."""
+    import oracle
+
+    specs = G.batch_instances()
+    assert len(specs) == 512
+    ns = [d["n"] for d in specs]
+    assert max(ns[:256]) >= 400 and min(ns[:256]) <= 20 and max(ns[256:]) >= 60
+    n_runs = 48
+    nets, qs, sims = [], [], []
+    try:
+        for d in specs:
+            net = cs.ParsedNetwork.from_string(d["xml"], d["loc_query"])
+            if d["kind"] == "sir":
+                q = cs.var_query(net.variable_id(d["var"]), d["time_bound"], threshold=d["threshold"])
+            else:
+                q = cs.reach_query(d["time_bound"])
+            nets.append(net)
+            qs.append(q)
+            sims.append(cs.GpuSimulator(net))
+        items = [(s, q, 900 + k, n_runs) for k, (s, q) in enumerate(zip(sims, qs))]
+        print(f"[c4] 512 instances x {n_runs} runs, one batch round", flush=True)
+        b = cs.Batch(items)
+        try:
+            res, st = b.run(1)
+        finally:
+            b.close()
+        assert st.runs == 512 * n_runs
+        print(f"[c4] groups={st.groups} graph_ms={st.graph_ms:.1f}; oracle", flush=True)
+        for k, (net, q, r) in enumerate(zip(nets, qs, res)):
+            o, _ = oracle.run(net, q, seed=900 + k, n_runs=n_runs, threads=16)
+            assert (r.hits, r.steps, r.transitions) == (o.hits, o.steps, o.transitions), (k, specs[k]["kind"], ns[k])
+            assert r.sum == o.sum, k
+    finally:
+        for s in sims:
+            s.close()
+
+
+def test_c4_instances_inside_ch_interval_of_reference_algorithm():
+    """The largest SIR and Fischer instances of the batch and two others at
+    1e5 runs: each estimate lies inside the Chernoff-Hoeffding interval of
+    the reference algorithm (the oracle's LITERAL sampling) on 3000 runs."""
+    import oracle
+
+    specs = G.batch_instances()
+    sir = max(range(256), key=lambda k: specs[k]["n"])
+    fis = 256 + max(range(256), key=lambda k: specs[256 + k]["n"])
+    for k in (sir, fis, 17, 300):
+        d = specs[k]
+        net = cs.ParsedNetwork.from_string(d["xml"], d["loc_query"])
+        q = (cs.var_query(net.variable_id(d["var"]), d["time_bound"], threshold=d["threshold"])
+             if d["kind"] == "sir" else cs.reach_query(d["time_bound"]))
+        sim = cs.GpuSimulator(net)
+        try:
+            r = sim.run(q, seed=41 + k, n_runs=100_000)
+        finally:
+            sim.close()
+        n_ref = 3000
+        ref, _ = oracle.run(net, q, seed=42 + k, n_runs=n_ref, mode=oracle.LITERAL, threads=16)
+        eps = cs.ch_epsilon(n_ref) + cs.ch_epsilon(100_000)
+        print(f"[c4 ch] instance {k} ({d['kind']} N={d['n']}): gpu {r.estimate:.4f} "
+              f"reference-algorithm {ref.estimate:.4f} eps {eps:.4f}", flush=True)
+        assert abs(r.estimate - ref.estimate) <= eps

@@ -17,6 +17,8 @@ from cusmc_amd import generators as G
 
 import oracle
 
+from _models import name_locations, receiver_goal
+
 pytestmark = pytest.mark.gpu
 
 PER_RUN = ("flag", "value", "steps", "transitions", "digest", "time")
@@ -32,6 +34,15 @@ def _net(models_dir, src, locq=""):
         xml, _ = G.fischer_network(int(arg))
     elif kind == "sir":
         xml, _ = G.sir_network(int(arg))
+    elif kind == "sirsmall":
+        xml, _ = G.sir_network(int(arg), 0.6, 0.2, initial_infected=2, seed=5)
+    elif kind == "recv":
+        # receiver-entered goal: alone (absorbing), beside a live component,
+        # beside a component whose rate reads a clock
+        live = None if arg == "" else float(arg.rstrip("c"))
+        xml = receiver_goal(live, clock_rate=arg.endswith("c"))
+    elif kind == "racetest_named":
+        xml = name_locations((models_dir / "RaceTest.xml").read_text(), {"id10": "sent", "id1": "got"})
     else:
         raise ValueError(src)
     return cs.ParsedNetwork.from_string(xml, locq)
@@ -60,6 +71,14 @@ CASES = [
     ("fischer32", "fischer:32", "P0.cs", ("reach", 300.0), 400, (0, 8, 16)),
     ("sir64", "sir:64", "", ("var", "I", 100.0, True, False, 20.0), 100, (0, 16)),
     ("sir350", "sir:350", "", ("var", "I", 100.0, True, False, 104.0), 24, (0,)),
+    # goals entered by broadcast receivers (the step after them decides)
+    ("recvgoal", "recv:", "", ("reach", 2.0), 4000, (0, 2)),
+    ("recvgoal_live", "recv:0.2", "", ("reach", 2.0), 4000, (0, 4)),
+    ("recvgoal_clockrate", "recv:0.3c", "", ("reach", 2.0), 4000, (0,)),
+    ("RaceTest_receiver_loc", "racetest_named:", "Process7.got", ("reach", 3.0), 4000, (0, 16)),
+    ("RaceTest_sender_loc", "racetest_named:", "Process4.sent", ("reach", 3.0), 2000, (0,)),
+    ("sir24_loc", "sirsmall:24", "A3.I", ("reach", 8.0), 2000, (0, 8)),
+    ("behavior3", "UppaalBehaviorTest3.xml", "c2.g2", ("reach", 20.0), 2000, (0,)),
 ]
 
 
@@ -96,7 +115,7 @@ def test_bit_exact_vs_oracle(case, models_dir):
 # Runs without digests may stage the model tables in LDS (workgroups of four
 # waves): the same per-run results as the oracle, for both register policies.
 TABLE_CASES = ["RaceTest", "UppaalWeightTest", "fischer_10_29", "aloha_25", "covid_32", "race8", "fischer32",
-               "sir64"]
+               "sir64", "recvgoal_live", "RaceTest_receiver_loc", "sir24_loc"]
 
 
 @pytest.mark.parametrize("name", TABLE_CASES)

@@ -19,7 +19,8 @@ from cusmc_amd import generators as G
 
 import oracle
 
-from _models import single_exponential, single_uniform, sir_exact, zeno
+from _models import (E_ADD, E_MUL, E_NOT, RawNetwork, random_tree, receiver_goal, receiver_goal_p, single_exponential,
+                     single_uniform, sir_exact, zeno)
 
 pytestmark = pytest.mark.gpu
 
@@ -62,6 +63,18 @@ def test_gpu_condtest_and_racetest(models_dir):
     mean = 8 * (1 - 2 ** -T) + (4 / 9) * 2 ** -T
     sd = math.sqrt(max(r.sumsq / n - (r.sum / n) ** 2, 0))
     assert abs(r.estimate - mean) <= 5 * sd / math.sqrt(n), (r.estimate, mean)
+
+
+@pytest.mark.parametrize("live", [None, 0.2], ids=["absorbing", "live"])
+def test_gpu_receiver_goal_closed_form(live):
+    """A goal entered by a broadcast receiver counts only if the next step of
+    every component keeps t < T (simulation.cu:409-415, 835-849)."""
+    net = cs.ParsedNetwork.from_string(receiver_goal(live), "")
+    n = 1_000_000
+    T = 2.0
+    r = _gpu(net, cs.reach_query(T), 26, n)
+    p = receiver_goal_p(T, live)
+    assert _within(r.estimate, p, n), (r.estimate, p)
 
 
 @pytest.mark.parametrize("n_agents,beta,gamma,k,T", [(12, 0.9, 0.3, 5, 4.0), (20, 0.6, 0.2, 8, 6.0)])
@@ -148,3 +161,52 @@ def test_gpu_bytecode_matches_tree_evaluation_on_random_expressions():
         assert np.array_equal(r.per_run["digest"], ro["digest"]), xml
         checked += 1
     assert checked >= 20
+
+
+def _full_operator_model(rng):
+    """Two components over four variables, built as network arrays so that
+    every operator of expressions.cu:10-95 appears: component 0 loops at rate
+    2 through two edges whose guards, weights and updates are random trees
+    (both edges enabled at once exercises the weighted pick); component 1
+    loops at rate 1 + 0.5 !(random tree), a rate expression that changes
+    with the variables."""
+    net = RawNetwork()
+    for k in range(4):
+        net.var(float(k + 1))
+    t = lambda d=3: random_tree(rng, net, 4, d)
+    l0 = net.component([{"rate": net.lit(2.0)}])
+    for _ in range(2):
+        guards = [(int(rng.choice([0, 1, 2, 3, 4])), int(rng.integers(0, 4)), t(2))] if rng.random() < 0.6 else []
+        upds = [(int(rng.integers(0, 4)), t()) for _ in range(int(rng.integers(1, 4)))]
+        net.edge(l0, l0, weight=t(2), guards=guards, upds=upds)
+    rate = net.expr(E_ADD, net.lit(1.0), net.expr(E_MUL, net.lit(0.5), net.expr(E_NOT, t())))
+    l1 = net.component([{"rate": rate}])
+    net.edge(l1, l1, upds=[(int(rng.integers(0, 4)), t())])
+    return net.build()
+
+
+def test_gpu_bytecode_matches_oracle_on_every_operator():
+    """Random trees over the whole operator set (sqrt, ^, &&, ||, !, ==, !=,
+    <, <=, >, >=, %, ?:, random, unary minus) in updates, guards, weights and
+    a rate: the trajectory digest holds every fired edge and the final
+    variable bits, so the GPU bytecode must agree with the oracle's tree
+    evaluation exactly."""
+    rng = np.random.default_rng(11)
+    checked = 0
+    for k in range(40):
+        net = _full_operator_model(rng)
+        try:
+            sim = cs.GpuSimulator(net)
+        except cs.SmcError:
+            continue  # beyond the bytecode's register or length limit
+        try:
+            q = cs.var_query(0, 3.0, is_max=True, estimate=True)
+            print(f"[gpu ops] model {k}", flush=True)
+            r = sim.run(q, seed=k, n_runs=64, per_run=("digest", "value", "transitions"))
+        finally:
+            sim.close()
+        o, ro = oracle.run(net, q, seed=k, n_runs=64, per_run=("digest", "value", "transitions"))
+        assert np.array_equal(r.per_run["transitions"], ro["transitions"]), k
+        assert np.array_equal(r.per_run["digest"], ro["digest"]), k
+        checked += 1
+    assert checked >= 30

@@ -20,7 +20,8 @@ from cusmc_amd import generators as G
 
 import oracle
 
-from _models import first_winner, single_exponential, single_uniform, sir_exact, urgent_order, zeno
+from _models import (clock_rate_race, first_winner, name_locations, receiver_goal, receiver_goal_p, single_exponential,
+                     single_uniform, sir_exact, urgent_order, zeno)
 
 MODES = [oracle.CANONICAL, oracle.LITERAL]
 MODE_IDS = ["canonical", "literal"]
@@ -192,3 +193,90 @@ def test_tree_pick_proportional_and_positive():
             assert rates[c] > 0
             hits[c] += 1
         assert np.allclose(hits / len(vs), rates / rates.sum(), atol=2e-3)
+
+
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+@pytest.mark.parametrize("live", [None, 0.2], ids=["absorbing", "live"])
+@pytest.mark.parametrize("T", [0.7, 2.0, 5.0])
+def test_receiver_goal_closed_form(mode, live, T):
+    """Q16: a goal entered by a broadcast receiver counts only if the step
+    after the receivers, raced by every component, keeps t < T
+    (simulation.cu:409-415 fire receivers inside the iteration, :835-849 test
+    the goal at the next loop top).  Without a live component the state is
+    absorbing once the receiver has fired; with one, the components that can
+    no longer fire still race in that step."""
+    net = cs.ParsedNetwork.from_string(receiver_goal(live), "")
+    n = 40000
+    res, _ = oracle.run(net, cs.reach_query(T), seed=13, n_runs=n, mode=mode, threads=8)
+    p = receiver_goal_p(T, live)
+    assert _prob_ok(res.estimate, p, n), (res.estimate, p)
+    # reading it as a winner-entered goal (hit iff the sender fires before T)
+    # is excluded, and so is dropping the receiver's hit
+    assert not _prob_ok(res.estimate, 1 - 2.0 ** -T, n, k=3.0)
+    assert res.estimate > 0.05
+
+
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+@pytest.mark.parametrize("T", [1.0, 3.0])
+def test_racetest_location_queries(mode, T, models_dir):
+    """RaceTest.xml with location queries: the sender's target (entered by
+    the race winner) is reached w.p. 1 - 2^-T; a receiver's target (entered
+    in the next iteration's broadcast) needs one more step of the nine
+    components at the default rate 1: 1 - 2^-T - (2^-T - 2^-9T)/8."""
+    xml = (models_dir / "RaceTest.xml").read_text()
+    xml = name_locations(xml, {"id10": "sent", "id1": "got"})
+    n = 40000
+    for query, p in (("Process4.sent", 1 - 2.0 ** -T),
+                     ("Process7.got", 1 - 2.0 ** -T - (2.0 ** -T - 2.0 ** (-9 * T)) / 8)):
+        net = cs.ParsedNetwork.from_string(xml, query)
+        res, _ = oracle.run(net, cs.reach_query(T), seed=14, n_runs=n, mode=mode, threads=8)
+        assert _prob_ok(res.estimate, p, n), (query, res.estimate, p)
+
+
+def test_clock_rate_races_every_component():
+    """A rate that reads a clock changes at every null step of cuSMC's race,
+    so the components that cannot fire still race: both samplings agree,
+    and the rate is not frozen at its first value (that would give
+    Pr = 1 - 2^(-0.1 T))."""
+    net = cs.ParsedNetwork.from_string(clock_rate_race(), "")
+    n = 30000
+    T = 4.0
+    a, _ = oracle.run(net, cs.reach_query(T), seed=15, n_runs=n, mode=oracle.CANONICAL, threads=8)
+    b, _ = oracle.run(net, cs.reach_query(T), seed=16, n_runs=n, mode=oracle.LITERAL, threads=8)
+    p = (a.estimate + b.estimate) / 2
+    se = math.sqrt(max(p * (1 - p), 1e-6) * 2 / n)
+    assert abs(a.estimate - b.estimate) <= 5 * se, (a.estimate, b.estimate)
+    frozen = 1 - 2 ** (-0.1 * T)
+    assert a.estimate > frozen + 0.1, (a.estimate, frozen)
+
+
+def _agree(net, q, n, seed=21):
+    a, _ = oracle.run(net, q, seed=seed, n_runs=n, mode=oracle.CANONICAL, threads=8)
+    b, _ = oracle.run(net, q, seed=seed + 1, n_runs=n, mode=oracle.LITERAL, threads=8)
+    p = (a.estimate + b.estimate) / 2
+    se = math.sqrt(max(p * (1 - p), 1e-6) * 2 / n)
+    assert abs(a.estimate - b.estimate) <= 5 * se, (a.estimate, b.estimate)
+    return a.estimate
+
+
+def test_location_queries_on_broadcast_models_agree_in_law(models_dir):
+    """Canonical sampling (absorption, left-out null racers, the crossing
+    correction) against cuSMC's per-component sampling, on location queries
+    of models with broadcasts: SIR agents reaching I as winner or receiver,
+    the receiver-goal model beside a clock-rate component, UppaalBehavior-
+    Test2 and fischermulti."""
+    n = 20000
+    xml, _ = G.sir_network(24, 0.6, 0.2, initial_infected=2, seed=5)
+    for query in ("A3.I", "A3.R"):
+        net = cs.ParsedNetwork.from_string(xml, query)
+        p = _agree(net, cs.reach_query(8.0), n)
+        assert 0.02 < p < 0.98, (query, p)
+    net = cs.ParsedNetwork.from_string(receiver_goal(0.3, clock_rate=True), "")
+    _agree(net, cs.reach_query(2.0), n)
+    # (UppaalBehaviorTest2's c1.f2 is left out: its runs are Zeno under
+    # these semantics, a uniform delay bounded by a shrinking invariant gap)
+    for fname, query, T in (("UppaalBehaviorTest2.xml", "c2.g2", 100.0),
+                            ("fischermulti.xml", "P1.cs", 20.0),
+                            ("UppaalBehaviorTest3.xml", "c2.g2", 20.0), ("UppaalBehaviorTest3.xml", "c1.f2", 5.0)):
+        net = cs.ParsedNetwork.from_file(models_dir / fname, query)
+        _agree(net, cs.reach_query(T), n)
