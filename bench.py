@@ -4,7 +4,10 @@ Default workload: Fischer mutual exclusion with 32 processes, query
 Pr[<=300](<> P0.cs), 1,000,000 runs per step, FP64 clocks and time.  One
 step = one smc_run() over all runs (run kernel + on-GPU reduction); the model
 tables are resident in HBM before the timed region, and the launch shape is
-picked by the autotuner before it.
+picked by the autotuner before it.  The default line also carries, under
+`learned_is`, the rare-event headline of the `is` workload below (wall time
+to +-10% with the learned sampler against plain simulation), measured in the
+same process after the timed steps.
 
 Other workloads (--workload):
   sir350   agent SIR, N=350, beta=0.3, gamma=0.1, Pr[<=100](<> I>=105), 1e6 runs
@@ -40,13 +43,20 @@ sys.path.insert(0, str(ROOT))
 LANE_PEAK = 256 * 64 * 2.4e9        # VALU lane-operations per second
 WAVE_PEAK = 256 * 2.4e9             # wave64 VALU instructions per second
 # algorithmic lane-ops per race step (oracle cost model; tests pin them)
-COST_PER_STEP = {"fischer32": 334.7, "sir350": 2626.3, "race8": 881.2}
+COST_PER_STEP = {"fischer32": 505.5, "sir350": 2626.0, "race8": 881.6}
 SEED = 20261202
-# HBM bytes per run moved by the run kernel (reads + writes), from rocprofv3
-# FETCH_SIZE / WRITE_SIZE counters scaled by what they read for a 1 GiB device
-# copy, for the launch shape (lanes per run, register policy, tables in LDS)
-# they were taken on; another shape reports no traffic
-TRAFFIC_PER_RUN = {"fischer32": ((4, 2, 1), 85.4), "sir350": ((64, 2, 0), 213.1)}
+# Counter figures of the run kernel from rocprofv3 runs of this file (the
+# commands and raw values are in profiles/), for the launch shape (lanes per
+# run, register policy, tables in LDS) they were taken on; another shape
+# reports null.  traffic: HBM bytes per run (FETCH_SIZE + WRITE_SIZE, scaled by
+# what they read for a 1 GiB device copy).  valu: SQ_INSTS_VALU (wave64 VALU
+# instructions) per race step.
+PROFILED = {
+    "fischer32": {"shape": (4, 2, 1), "traffic_per_run": None, "valu_per_step": None,
+                  "source": "profiles/sim_kernel_fischer32.md"},
+    "sir350": {"shape": (64, 2, 0), "traffic_per_run": None, "valu_per_step": None,
+               "source": "profiles/sim_kernel_sir350.md"},
+}
 
 
 def _models(workload):
@@ -94,8 +104,11 @@ def bench_plain(args) -> dict:
     kms = sum(r.kernel_ms for r in results)
     achieved = COST_PER_STEP[args.workload] * steps / (kms / 1e3)
     used = (results[-1].lanes_per_run, policy, int(bool(tables)))
-    shape_t, per_run_t = TRAFFIC_PER_RUN.get(args.workload, (None, None))
-    traffic = per_run_t * runs if shape_t == used else None
+    prof = PROFILED.get(args.workload, {})
+    same_shape = prof.get("shape") == used
+    traffic = prof["traffic_per_run"] * runs if same_shape and prof.get("traffic_per_run") else None
+    valu_frac = (prof["valu_per_step"] * steps / (kms / 1e3) / WAVE_PEAK
+                 if same_shape and prof.get("valu_per_step") else None)
     line = {
         "metric": "runs/sec; transitions/sec reported alongside",
         "value": n / wall,
@@ -121,10 +134,25 @@ def bench_plain(args) -> dict:
         "roofline": {"bound": "valu", "achieved": achieved, "peak": LANE_PEAK, "unit": "lane-op/s",
                      "frac": achieved / LANE_PEAK, "traffic": traffic,
                      "work": f"{COST_PER_STEP[args.workload]} algorithmic lane-ops per race step x "
-                             f"{steps // args.steps} race steps per launch"},
+                             f"{steps // args.steps} race steps per launch",
+                     # issued wave64 VALU instructions / (kernel time x 256 CUs x 2.4 GHz)
+                     "valu_issue_frac": valu_frac,
+                     "counters_source": (prof.get("source") + " (rocprofv3 on this launch shape)")
+                     if same_shape and (traffic or valu_frac) else None},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(net, q, args)
+    if args.workload == "fischer32" and not args.no_is:
+        # the learned importance sampler's headline (rare-event SIR-100)
+        _progress("learned IS leg")
+        r = bench_is(args)
+        line["learned_is"] = {k: r[k] for k in (
+            "is_seconds", "is_train_seconds", "is_estimate", "is_rel_half_width", "plain_seconds_to_ci",
+            "plain_estimate", "plain_rel_half_width", "plain_runs_per_sec", "plain_extrapolated",
+            "is_inside_plain_interval")}
+        line["learned_is"]["speedup"] = r["value"]
+        line["learned_is"]["workload"] = r["config"]["workload"]
+        line["learned_is"]["exact"] = r["config"]["exact"]
     return line
 
 
@@ -260,26 +288,33 @@ def _cpu_model() -> str:
 
 
 def cpu_baseline(net, q, args) -> dict:
-    """The oracle (cuSMC's per-component race, literal mode) on the host
-    cores, on a bounded sample of the same workload: all cores the process
-    may use, and one core."""
+    """The oracle on the host cores, on a bounded sample of the same
+    workload: cuSMC's per-component race (literal mode: the reported
+    baseline) on all cores the process may use and on one core, and beside
+    it the GPU's own sampling of the race (canonical mode) on all cores, so
+    the GPU/CPU ratio can be read with the algorithm held fixed."""
     import oracle
 
-    def timed(threads, seconds):
+    def timed(threads, seconds, mode):
         n = max(threads, 64)
-        res, _ = oracle.run(net, q, seed=SEED, n_runs=n, mode=oracle.LITERAL, threads=threads)
+        res, _ = oracle.run(net, q, seed=SEED, n_runs=n, mode=mode, threads=threads)
         rate = n / max(res.seconds, 1e-9)
         n = max(threads, int(rate * seconds))
-        res, _ = oracle.run(net, q, seed=SEED, n_runs=n, mode=oracle.LITERAL, threads=threads)
+        res, _ = oracle.run(net, q, seed=SEED, n_runs=n, mode=mode, threads=threads)
         return n, res
     threads = _host_cores()
-    n, res = timed(threads, args.cpu_seconds)
-    n1, res1 = timed(1, max(2.0, args.cpu_seconds / 3))
-    _progress(f"cpu baseline: {n / res.seconds:.1f} runs/s on {res.threads} threads")
+    n, res = timed(threads, args.cpu_seconds, oracle.LITERAL)
+    n1, res1 = timed(1, max(2.0, args.cpu_seconds / 3), oracle.LITERAL)
+    nc, resc = timed(threads, max(2.0, args.cpu_seconds / 3), oracle.CANONICAL)
+    _progress(f"cpu baseline: {n / res.seconds:.1f} runs/s on {res.threads} threads "
+              f"(canonical sampling {nc / resc.seconds:.1f} runs/s)")
     return {"value": n / res.seconds, "unit": "runs/s", "cores": res.threads, "kind": "port",
             "sample": f"{n} runs of the same workload, oracle literal mode, {res.seconds:.1f} s on "
                       f"{res.threads} threads ({_cpu_model()})",
-            "single_core": {"value": n1 / res1.seconds, "unit": "runs/s", "sample": f"{n1} runs, {res1.seconds:.1f} s"}}
+            "single_core": {"value": n1 / res1.seconds, "unit": "runs/s", "sample": f"{n1} runs, {res1.seconds:.1f} s"},
+            "canonical": {"value": nc / resc.seconds, "unit": "runs/s", "cores": resc.threads,
+                          "sample": f"{nc} runs, oracle canonical mode (the GPU's sampling of the race), "
+                                    f"{resc.seconds:.1f} s"}}
 
 
 def bench_copy(args) -> dict:
@@ -322,6 +357,7 @@ def main() -> int:
     ap.add_argument("--tables", type=int, default=0, choices=[0, 1],
                     help="with --lanes: model tables in global memory (0) or staged in LDS (1)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-is", action="store_true", help="fischer32: skip the learned-IS leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not run here (the run range would be split "

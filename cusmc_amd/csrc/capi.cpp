@@ -270,6 +270,8 @@ int smc_model_create(const smc_network *net, int device, smc_model **out) {
         d.n_leaves = im.n_leaves;
         d.n_locs = (int)im.locs.size();
         d.n_edges = net->n_edges;
+        d.freeze = im.freeze ? 1 : 0;
+        d.guarded = im.guarded ? 1 : 0;
         hip_check(hipMalloc(&m->d_inst, sizeof(InstDesc)), "hipMalloc instance");
         hip_check(hipMalloc(&m->d_counter, sizeof(unsigned long long)), "hipMalloc counter");
         hip_check(hipMalloc(&m->d_partials, 1024 * sizeof(ReducePartial)), "hipMalloc partials");

@@ -243,6 +243,17 @@ ModelImage compile_model(const smc_network &n) {
         return vu.set == 0 || (vu.add == 0 && vu.one_value);
     };
     std::vector<uint8_t> fast_edge(n.n_edges, 0);
+    std::vector<int8_t> clock_memo(n.n_exprs, -1);
+    std::function<bool(int)> reads_clock = [&](int x) -> bool {
+        if (x < 0) return false;
+        if (clock_memo[x] >= 0) return clock_memo[x] != 0;
+        const smc_expr_node &nd = n.exprs[x];
+        bool r = nd.op == SMC_E_VAR && n.var_kind[nd.var] == SMC_VAR_CLOCK;
+        if (!r && nd.op != SMC_E_LIT && nd.op != SMC_E_VAR)
+            r = reads_clock(nd.a) || reads_clock(nd.b) || reads_clock(nd.c);
+        clock_memo[x] = r ? 1 : 0;
+        return r;
+    };
 
     for (int e = 0; e < n.n_edges; ++e) {
         const smc_edge &E = n.edges[e];
@@ -260,6 +271,7 @@ ModelImage compile_model(const smc_network &n) {
             DevGuard g{};
             g.var = (uint16_t)C.var;
             g.op = (uint8_t)C.op;
+            g.clock_free = n.var_kind[C.var] != SMC_VAR_CLOCK && !reads_clock(C.rhs);
             g.rhs = lw.ref(C.rhs, 0.0);
             img.guards.push_back(g);
         }
@@ -295,17 +307,6 @@ ModelImage compile_model(const smc_network &n) {
     // time_free: the location's rate and its spontaneous edges' guards read
     // no clock, so between transitions neither its enabled edges nor its
     // rate change
-    std::vector<int8_t> clock_memo(n.n_exprs, -1);
-    std::function<bool(int)> reads_clock = [&](int x) -> bool {
-        if (x < 0) return false;
-        if (clock_memo[x] >= 0) return clock_memo[x] != 0;
-        const smc_expr_node &nd = n.exprs[x];
-        bool r = nd.op == SMC_E_VAR && n.var_kind[nd.var] == SMC_VAR_CLOCK;
-        if (!r && nd.op != SMC_E_LIT && nd.op != SMC_E_VAR)
-            r = reads_clock(nd.a) || reads_clock(nd.b) || reads_clock(nd.c);
-        clock_memo[x] = r ? 1 : 0;
-        return r;
-    };
     // Null racers leave an all-exponential race only while every rate is
     // constant between transitions (a rate reading a clock changes at each
     // null step of the reference's race).
@@ -324,19 +325,28 @@ ModelImage compile_model(const smc_network &n) {
                 if (n.var_kind[G.var] == SMC_VAR_CLOCK || reads_clock(G.rhs)) tf = false;
             }
         }
-        bool dead = img.locs[l].klass == SMC_K_EXP && !clock_rate;
-        for (int k = 0; k < L.edge_count && dead; ++k) {
+        // blocking: always (every spontaneous edge has `!=`, or none), by
+        // the variables (every other spontaneous edge has a clock-free
+        // conjunct that may be false), or never
+        int kind = 1;
+        for (int k = 0; k < L.edge_count && kind != 0; ++k) {
             const smc_edge &E = n.edges[L.edge_first + k];
             if (E.channel < 0) continue;
-            bool never = false;
+            bool never = false, clock_free = false;
             for (int g = 0; g < E.guard_count; ++g) {
                 const smc_constraint &G = n.constraints[E.guard_first + g];
-                if (G.uses_variable && G.op == SMC_C_NE) never = true;
+                if (!G.uses_variable) continue;
+                if (G.op == SMC_C_NE) never = true;
+                else if (n.var_kind[G.var] != SMC_VAR_CLOCK && !reads_clock(G.rhs)) clock_free = true;
             }
-            if (!never) dead = false;
+            if (never) continue;
+            kind = clock_free ? 2 : 0;
         }
-        img.locs[l].flags = (uint8_t)((tf ? SMC_LF_TIME_FREE : 0) | (dead ? SMC_LF_DEAD : 0));
+        img.locs[l].flags = (uint8_t)((tf ? SMC_LF_TIME_FREE : 0) | (kind == 1 ? SMC_LF_DEAD : 0) |
+                                      (kind == 2 ? SMC_LF_GUARDED : 0));
+        if (kind == 2 && img.locs[l].klass != SMC_K_ZERO) img.guarded = true;
     }
+    img.freeze = !clock_rate;
     for (int l = 0; l < n.n_locations; ++l) {
         const smc_location &L = n.locations[l];
         uint32_t bits = 0;

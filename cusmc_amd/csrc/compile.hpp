@@ -22,6 +22,8 @@ struct ModelImage {
     std::vector<uint16_t> clocks;
     std::vector<int32_t> var_kind;
     int n_comp = 0, n_vars = 0, n_leaves = 1;
+    bool freeze = true;   // no location's rate reads a clock
+    bool guarded = false; // some location is blocked or not by the variables
 };
 
 // Throws std::runtime_error (message prefixed "limit:" for size limits).

@@ -186,6 +186,8 @@ __device__ __forceinline__ DevModelT<SmcLdsTables> lds_model(const DevModel &mg,
     m.n_leaves = mg.n_leaves;
     m.n_locs = mg.n_locs;
     m.n_edges = mg.n_edges;
+    m.freeze = mg.freeze;
+    m.guarded = mg.guarded;
     return m;
 }
 
@@ -312,11 +314,12 @@ __device__ __forceinline__ int pick_edge(uint64_t mask, int edge_first, uint64_t
     return lowest_lane(mask);
 }
 
-// Rate sum, on the race tree, of the components that cannot fire
-// (SMC_LF_DEAD) in the slots of deadmask.  They leave the race when every
-// racer is exponential; the transition that crosses the time bound still
-// competes with their first null event after the bound, which is drawn from
-// this sum (memoryless races).
+// Rate sum, on the race tree, of the components that cannot fire before
+// the next transition (frozen: blocked, exponential now) in the slots of
+// deadmask.  They leave the race when every racer is exponential; the
+// transition that crosses the time bound still competes with their first
+// null event after the bound, which is drawn from this sum (memoryless
+// races).
 template <int C, class M>
 __device__ __forceinline__ double dead_total(uint32_t deadmask, const int (&loc)[C], const double *vars,
                                              const M &m, int S) {
@@ -359,15 +362,51 @@ __device__ __forceinline__ int fire_edge(int e, double *vars, const M &m, LaneQu
     return E.dst;
 }
 
+// Slots (bit j) whose component is blocked by the variables: in a
+// location flagged SMC_LF_GUARDED (not zero-delay), every spontaneous edge
+// has a `!=` conjunct or a false conjunct that reads no clock.  A blocked
+// component cannot fire until a transition changes the variables; the
+// kernels recompute the mask after every change of state, on models with
+// such locations (DevModel.guarded).  Locations flagged SMC_LF_DEAD are
+// always blocked and are tested by their flag.
+template <int C, class M>
+__device__ __forceinline__ uint32_t blocked_slots(const int (&loc)[C], const double *vars, const M &m, int sl,
+                                                  int n) {
+    uint32_t b = 0;
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        const int c = sl * C + j;
+        if (c >= n) continue;
+        const DevLoc L = load_rec(m.locs + loc[j]);
+        bool blk = false;
+        if ((L.flags & SMC_LF_GUARDED) && L.klass != SMC_K_ZERO) {
+            blk = true;
+            for (int i = 0; i < L.edge_n && blk; ++i) {
+                const DevEdge E = load_rec(m.edges + L.edge_first + i);
+                if (E.never || E.channel < 0) continue;
+                bool e_blk = false;
+                for (int g = 0; g < E.guard_n && !e_blk; ++g) {
+                    const DevGuard G = load_rec(m.guards + E.guard_first + g);
+                    if (G.clock_free) e_blk = !guard_holds(G.op, vars[G.var], eval_ref(G.rhs, vars, m));
+                }
+                blk = e_blk;
+            }
+        }
+        if (blk) b |= 1u << j;
+    }
+    return b;
+}
+
 // Whether no component of the segment's run can fire again before the time
-// bound: each one is in a location whose rate and spontaneous guards read no
-// clock, and it has no enabled spontaneous edge or races at rate 0 (an
-// exponential location).  Zero-delay and clock-bounded locations with an
-// enabled edge can fire.  Nothing but time changes such a state, so the run's
-// outcome is decided; the kernels check it after every change of state.
+// bound: each one is blocked (bit of `blk`), or is in a location whose rate
+// and spontaneous guards read no clock and has no enabled spontaneous edge
+// or races at rate 0 (an exponential location).  Zero-delay and
+// clock-bounded locations with an enabled edge can fire.  Nothing but time
+// changes such a state, so the run's outcome is decided; the kernels check
+// it after every change of state.
 template <int C, class M>
 __device__ __forceinline__ bool absorbed(const int (&loc)[C], const double *vars, const M &m, int sl, int n,
-                                         uint64_t seg_mask) {
+                                         uint64_t seg_mask, uint32_t blk) {
     bool can = false;
     RateMemo memo;
 #pragma unroll
@@ -375,7 +414,9 @@ __device__ __forceinline__ bool absorbed(const int (&loc)[C], const double *vars
         const int c = sl * C + j;
         if (can || c >= n) continue;
         const DevLoc L = load_rec(m.locs + loc[j]);
-        if (!(L.flags & SMC_LF_TIME_FREE) || L.klass == SMC_K_ZERO) { can = true; continue; }
+        if (L.klass == SMC_K_ZERO) { can = true; continue; }
+        if ((L.flags & SMC_LF_DEAD) || ((blk >> j) & 1u)) continue;
+        if (!(L.flags & SMC_LF_TIME_FREE)) { can = true; continue; }
         if (enabled_mask(loc[j], 0, vars, m) == 0) continue;
         if (L.klass != SMC_K_EXP) { can = true; continue; }
         const double r = location_rate(L, vars, m, memo);

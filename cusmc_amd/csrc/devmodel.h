@@ -48,8 +48,10 @@ enum smc_delay_class {
 enum {
     SMC_LF_TIME_FREE = 1,   // rate and spontaneous guards read no clock: enabling and rate
                             // change only with transitions
-    SMC_LF_DEAD = 2         // exponential, and no spontaneous edge can ever fire: its races
-                            // are null steps
+    SMC_LF_DEAD = 2,        // no spontaneous edge can ever fire (each has a `!=` conjunct, or
+                            // there is none): always blocked
+    SMC_LF_GUARDED = 4      // blocked or not by the variables: every spontaneous edge without
+                            // `!=` has a conjunct that reads no clock
 };
 
 struct DevInsn {
@@ -97,7 +99,7 @@ struct DevEdge {            // 32 bytes
 struct DevGuard {           // variables[var] op rhs
     uint16_t var;
     uint8_t op;             // smc_cons_op, never SMC_C_NE
-    uint8_t pad;
+    uint8_t clock_free;     // neither var nor rhs reads a clock: fixed between transitions
     uint32_t rhs;
 };
 
@@ -156,5 +158,7 @@ struct DevModelT {
     int32_t n_leaves;           // next power of two >= n_comp
     int32_t n_locs;
     int32_t n_edges;
+    int32_t freeze;             // no rate reads a clock: blocked components may leave the race
+    int32_t guarded;            // some location is flagged SMC_LF_GUARDED
 };
 using DevModel = DevModelT<SmcGlobalTables>;

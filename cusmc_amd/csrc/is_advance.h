@@ -170,6 +170,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
     bool first_race = true;
     bool hit = false, end = false, fired_any = false;
     bool changed = true;  // state changed since the last absorption check
+    uint32_t blk = 0;     // blocked slots at the last change of state
 
     for (;;) {
         if (!mid) {
@@ -214,10 +215,14 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                 // a goal entered by receivers: this step is raced in full
                 // and decides the hit (as in the run kernel)
                 const bool full = P.reach && goal;
-                if (changed && !full) {
-                    changed = false;
-                    if (absorbed<C>(loc, vars, m, sl, n, seg_mask)) { end = true; break; }
+                if (changed) {
+                    if (m.guarded) blk = blocked_slots<C>(loc, vars, m, sl, n);
+                    if (!full) {
+                        changed = false;
+                        if (absorbed<C>(loc, vars, m, sl, n, seg_mask, blk)) { end = true; break; }
+                    }
                 }
+
                 double leaf[C], leaft[C], Mb[C];
                 RateMemo memo;
                 int zero_j = C;
@@ -243,7 +248,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                             if (bounded) {
                                 if (M == 0.0) { if (zero_j == C) zero_j = j; }
                                 else { Mb[j] = M; bmask |= 1u << j; }
-                            } else if (L.flags & SMC_LF_DEAD) {
+                            } else if (m.freeze && ((L.flags & SMC_LF_DEAD) || ((blk >> j) & 1u))) {
                                 deadmask |= 1u << j;
                             } else {
                                 const double r = location_rate(L, vars, m, memo);
@@ -257,8 +262,8 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                         }
                     }
                 }
-                // components that cannot fire race only beside a uniform
-                // racer or in a full race (as in the run kernel)
+                // frozen components race only beside a uniform racer or in
+                // a full race (as in the run kernel)
                 const bool uniform = (__ballot(bmask != 0) & seg_mask) != 0;
                 if (uniform || full) {
                     for (uint32_t b = deadmask; b; b &= b - 1) {

@@ -118,6 +118,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
     int chan = 0, sender = -1;
     bool goal = false, goal_local = false;
     bool changed = true;  // state changed since the last absorption check
+    uint32_t blk = 0;     // blocked slots at the last change of state
     uint64_t digest = 0;
     LaneQuery q{p.qvar, p.qinit, p.qinit};
 
@@ -169,10 +170,14 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                     // t < bound (simulation.cu:409-415, 835-849): no
                     // absorption exit and no left-out null racers then
                     const bool full = p.reach && goal;
-                    if (changed && !full) {
-                        changed = false;
-                        if (absorbed<C>(loc, vars, m, sl, n, seg_mask)) { end = true; break; }
+                    if (changed) {
+                        if (m.guarded) blk = blocked_slots<C>(loc, vars, m, sl, n);
+                        if (!full) {
+                            changed = false;
+                            if (absorbed<C>(loc, vars, m, sl, n, seg_mask, blk)) { end = true; break; }
+                        }
                     }
+
                     double leaf[C], Mb[C];
                     int zero_j = C;
                     uint32_t bmask = 0, deadmask = 0;
@@ -199,7 +204,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                                 if (bounded) {
                                     if (M == 0.0) { if (zero_j == C) zero_j = j; }
                                     else { Mb[j] = M; bmask |= 1u << j; }
-                                } else if (L.flags & SMC_LF_DEAD) {
+                                } else if (m.freeze && ((L.flags & SMC_LF_DEAD) || ((blk >> j) & 1u))) {
                                     deadmask |= 1u << j;
                                 } else {
                                     const double r = location_rate(L, vars, m, memo);
@@ -208,12 +213,13 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                             }
                         }
                     }
-                    // A component whose location cannot fire only makes null
-                    // steps.  With every racer exponential, leaving it out of
-                    // the race keeps the law of everything else (memoryless
-                    // races); a uniform racer is redrawn at every step, so then
-                    // it races as in the reference, as it does when the step
-                    // decides the outcome (full).
+                    // A frozen component (blocked, exponential) only makes
+                    // null steps until the next transition.  With every racer
+                    // exponential, leaving it out of the race keeps the law of
+                    // everything else (memoryless races, rates constant between
+                    // transitions); a uniform racer is redrawn at every step,
+                    // so then it races as in the reference, as it does when
+                    // the step decides the outcome (full).
                     const bool uniform = (__ballot(bmask != 0) & seg_mask) != 0;
                     if (uniform || full) {
                         for (uint32_t b = deadmask; b; b &= b - 1) {

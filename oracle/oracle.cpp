@@ -24,9 +24,11 @@
 //   a run is cut off after max_steps race rounds (no counterpart: cuSMC
 //     spins forever on a cycle of zero delays)
 //   CANONICAL only: a run ends when no component can fire again before the
-//     bound (absorption: every component has no enabled spontaneous edge or
-//     races at rate 0, in a location whose rate and guards read no clock);
-//     cuSMC races null steps to the bound there, with the same outcome
+//     bound (absorption: every component is blocked — each spontaneous edge
+//     has a false conjunct that reads no clock — or has no enabled
+//     spontaneous edge or races at rate 0, in a location whose rate and
+//     guards read no clock); cuSMC races null steps to the bound there, with
+//     the same outcome
 //   guards: only conjuncts on a variable; `==` within 1e-5; `!=` disables
 //     the edge                                          simulation.cu:184-266
 //
@@ -36,6 +38,10 @@
 //     first (lowest index); the exponential components form one race, total
 //     rate on a fixed pairwise tree, delay -log2(u1)/total, winner by
 //     descending the tree with u2*total; bounded components draw uniforms.
+//     Components that can only make null steps until the next transition
+//     (frozen: exponential, every spontaneous edge blocked by a clock-free
+//     conjunct) leave an all-exponential race; the step that crosses the
+//     bound draws their first event after the bound.
 //     Same Philox stream, same operations, same bits as the GPU.
 // Resolutions of undefined behaviour (by intent): all variables and all
 // clocks are tracked (cuSMC: 3), a receiver picks with its own stream,
@@ -196,7 +202,9 @@ public:
         // upper bounds present (the conditions of absorption below)
         time_free_.assign(n.n_locations, 1);
         clock_bounded_.assign(n.n_locations, 0);
-        dead_.assign(n.n_locations, 0);
+        blocked_.assign(n.n_components, 0);
+        null_.assign(n.n_components, 0);
+        blk_kind_.assign(n.n_locations, 0);
         for (int l = 0; l < n.n_locations; ++l) {
             const smc_location &L = n.locations[l];
             if (L.rate >= 0 && reads_clock(L.rate)) time_free_[l] = 0;
@@ -214,29 +222,31 @@ public:
                 if (I.uses_variable && n.var_kind[I.var] == SMC_VAR_CLOCK && (I.op == SMC_C_LE || I.op == SMC_C_LT))
                     clock_bounded_[l] = 1;
             }
-            // dead: exponential, and every spontaneous edge has a `!=` guard
-            // (never enabled) or there is none
-            bool dead = L.type <= SMC_LOC_GOAL && !clock_bounded_[l];
-            for (int k = 0; k < L.edge_count && dead; ++k) {
+            // blocking: 1 always (every spontaneous edge has a `!=` conjunct,
+            // or there is none), 2 depends on the variables (a spontaneous
+            // edge without `!=` has a conjunct that reads no clock), 0 never
+            int kind = 1;
+            for (int k = 0; k < L.edge_count && kind != 0; ++k) {
                 const smc_edge &E = n.edges[L.edge_first + k];
                 if (E.channel < 0) continue;
-                bool never = false;
+                bool never = false, clock_free = false;
                 for (int g = 0; g < E.guard_count; ++g) {
                     const smc_constraint &G = n.constraints[E.guard_first + g];
-                    if (G.uses_variable && G.op == SMC_C_NE) never = true;
+                    if (!G.uses_variable) continue;
+                    if (G.op == SMC_C_NE) never = true;
+                    else if (n.var_kind[G.var] != SMC_VAR_CLOCK && !reads_clock(G.rhs)) clock_free = true;
                 }
-                if (!never) dead = false;
+                if (never) continue;
+                kind = clock_free ? 2 : 0;
             }
-            dead_[l] = dead ? 1 : 0;
+            blk_kind_[l] = (uint8_t)kind;
         }
         // Leaving null racers out of an all-exponential race keeps the law
         // only while every rate is constant between transitions: a rate that
         // reads a clock is re-evaluated at each of cuSMC's null steps
         // (simulation.cu:522-530), so then every component races.
-        bool clock_rate = false;
         for (int l = 0; l < n.n_locations; ++l)
-            if (n.locations[l].rate >= 0 && reads_clock(n.locations[l].rate)) clock_rate = true;
-        if (clock_rate) std::fill(dead_.begin(), dead_.end(), (uint8_t)0);
+            if (n.locations[l].rate >= 0 && reads_clock(n.locations[l].rate)) freeze_ = false;
     }
 
     // events: optional log of (step, component, edge) per fired edge
@@ -343,9 +353,14 @@ public:
             const bool goal_pending = reach && goal;
             // absorption (canonical sampling; cuSMC keeps racing null
             // steps to the bound, with the same outcome)
-            if (mode_ == ORACLE_CANONICAL && changed && !goal_pending) {
-                changed = false;
-                if (absorbed()) break;
+            if (mode_ == ORACLE_CANONICAL && changed) {
+                // blocked components: unchanged until the next transition
+                for (int c = 0; c < N; ++c)
+                    blocked_[c] = n_.locations[loc_[c]].type <= SMC_LOC_GOAL && blocked(c);
+                if (!goal_pending) {
+                    changed = false;
+                    if (absorbed()) break;
+                }
             }
 
             double d;
@@ -408,7 +423,11 @@ private:
     int qvar_ = -1;
     double qmax_ = 0, qmin_ = 0;
 
-    std::vector<uint8_t> time_free_, clock_bounded_, dead_;
+    std::vector<uint8_t> time_free_, clock_bounded_;
+    std::vector<uint8_t> blocked_;  // per component, at the last change of state
+    std::vector<uint8_t> blk_kind_; // per location: blocked never (0), always (1), by the variables (2)
+    std::vector<uint8_t> null_;     // per component: left out of this race
+    bool freeze_ = true;            // no rate reads a clock
 
     bool reads_clock(int x) const {
         if (x < 0) return false;
@@ -418,14 +437,51 @@ private:
         return reads_clock(nd.a) || reads_clock(nd.b) || reads_clock(nd.c);
     }
 
-    // No component can fire before the time bound: each sits in a location
-    // whose rate and spontaneous guards read no clock, with no enabled
-    // spontaneous edge or an exponential rate of 0.  Only time changes such
-    // a state, so the outcome is decided.
+    // Every spontaneous edge of component c has a false conjunct that reads
+    // no clock (or a `!=` conjunct, never enabled): c cannot fire until a
+    // transition changes the variables.
+    bool blocked(int c) const {
+        const int kind = blk_kind_[loc_[c]];
+        if (kind != 2) return kind == 1;
+        const smc_location &L = n_.locations[loc_[c]];
+        for (int k = 0; k < L.edge_count; ++k) {
+            const smc_edge &E = n_.edges[L.edge_first + k];
+            cost(2 * COST_OP);
+            if (E.channel < 0) continue;
+            bool b = false;
+            for (int g = 0; g < E.guard_count && !b; ++g) {
+                const smc_constraint &G = n_.constraints[E.guard_first + g];
+                if (!G.uses_variable) continue;
+                cost(2 * COST_OP);
+                if (G.op == SMC_C_NE) { b = true; break; }
+                if (n_.var_kind[G.var] == SMC_VAR_CLOCK || reads_clock(G.rhs)) continue;
+                const double x = vars_[G.var];
+                const double r = eval(n_, G.rhs, vars_.data());
+                bool ok;
+                switch (G.op) {
+                case SMC_C_LT: ok = x < r; break;
+                case SMC_C_LE: ok = x <= r; break;
+                case SMC_C_GT: ok = x > r; break;
+                case SMC_C_GE: ok = x >= r; break;
+                default: ok = std::fabs(x - r) <= TOL; break;  // SMC_C_EQ
+                }
+                b = !ok;
+            }
+            if (!b) return false;
+        }
+        return true;
+    }
+
+    // No component can fire before the time bound: each is blocked, or sits
+    // in a location whose rate and spontaneous guards read no clock, with no
+    // enabled spontaneous edge or an exponential rate of 0.  Only time
+    // changes such a state, so the outcome is decided.
     bool absorbed() const {
         for (int c = 0; c < n_.n_components; ++c) {
             const int l = loc_[c];
-            if (n_.locations[l].type > SMC_LOC_GOAL || !time_free_[l]) return false;
+            if (n_.locations[l].type > SMC_LOC_GOAL) return false;
+            if (blocked_[c]) continue;
+            if (!time_free_[l]) return false;
             std::vector<int> en;
             enabled(l, 0, &en);
             if (en.empty()) continue;
@@ -542,6 +598,7 @@ private:
         bool uniform = false;
         for (int c = 0; c < N; ++c) {
             leaf_[c] = 0.0;
+            null_[c] = 0;
             const int type = n_.locations[loc_[c]].type;
             if (type > SMC_LOC_GOAL) { if (zero < 0) zero = c; continue; }
             double M;
@@ -552,7 +609,9 @@ private:
                 const double d = M * smc_u01(w.v[0], w.v[1]);
                 if (d < du) { du = d; wu = c; }
                 uniform = true;
-            } else if (!dead_[loc_[c]]) {
+            } else if (freeze_ && blocked_[c]) {
+                null_[c] = 1;  // frozen
+            } else {
                 const double r = rate_of(c);
                 leaf_[c] = r > 0.0 ? r : 0.0;
             }
@@ -564,7 +623,7 @@ private:
         // the step itself decides the outcome (full).
         if (uniform || full)
             for (int c = 0; c < N; ++c)
-                if (n_.locations[loc_[c]].type <= SMC_LOC_GOAL && dead_[loc_[c]] && !clock_bounded_[loc_[c]]) {
+                if (null_[c]) {
                     const double r = rate_of(c);
                     leaf_[c] = r > 0.0 ? r : 0.0;
                 }
@@ -608,7 +667,7 @@ private:
             bool any = false;
             dleaf_.assign(N, 0.0);
             for (int c = 0; c < N; ++c)
-                if (n_.locations[loc_[c]].type <= SMC_LOC_GOAL && dead_[loc_[c]]) {
+                if (null_[c]) {
                     const double r = rate_of(c);
                     dleaf_[c] = r > 0.0 ? r : 0.0;
                     any = true;
