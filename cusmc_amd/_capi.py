@@ -119,6 +119,7 @@ SIGNATURES = {
     "smc_model_create": (i32, [P(Network), C.c_int, P(C.c_void_p)]),
     "smc_model_destroy": (None, [C.c_void_p]),
     "smc_run": (i32, [C.c_void_p, P(Query), u64, u64, u64, P(Launch), P(RunOutputs), P(Result)]),
+    "smc_run_noalloc": (i32, [C.c_void_p, P(Query), u64, u64, u64, P(Launch), P(Result)]),
     "smc_autotune": (i32, [C.c_void_p, P(Query), u64, u64, P(Launch)]),
     "smc_batch_create": (i32, [P(BatchItem), C.c_int, C.c_int, P(C.c_void_p)]),
     "smc_batch_run": (i32, [C.c_void_p, C.c_int, P(Result), P(BatchStats)]),

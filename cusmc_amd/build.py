@@ -28,7 +28,7 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", 
              "-Wno-unused-command-line-argument"]
 
 HOST_SOURCES = ["network.cpp", "uppaal.cpp", "compile.cpp"]
-HIP_SOURCES = ["capi.cpp", "is.cpp", "sim_kernel.hip", "is_kernels.hip"]
+HIP_SOURCES = ["capi.cpp", "is.cpp", "sim_kernel.hip", "is_kernels.hip", "noalloc.hip"]
 # kernels compiled once per (component slots per lane, register policy)
 VARIANT_SOURCES = ["sim_variant.hip", "is_variant.hip"]
 VARIANTS = {"sim_variant.hip": [(c, o) for c in (1, 2, 4, 8, 16) for o in (0, 2)],

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "capi_internal.hpp"
+#include "noalloc.hpp"
 #include "uppaal.hpp"
 
 namespace smc::capi {
@@ -342,6 +343,85 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
             if (outputs->digest) hip_check(hipMemcpyAsync(outputs->digest, b.digest, n_runs * 8, hipMemcpyDeviceToHost, m->stream), "copy digests");
             if (outputs->time) hip_check(hipMemcpyAsync(outputs->time, b.time, n_runs * 8, hipMemcpyDeviceToHost, m->stream), "copy times");
         }
+        hip_check(hipStreamSynchronize(m->stream), "run");
+        float k_ms = 0.f, r_ms = 0.f;
+        hip_check(hipEventElapsedTime(&k_ms, m->ev[0], m->ev[1]), "event time");
+        hip_check(hipEventElapsedTime(&r_ms, m->ev[1], m->ev[2]), "event time");
+        fill_result(q, m->h_acc, n_runs, out);
+        out->kernel_ms = k_ms;
+        out->reduce_ms = r_ms;
+        out->lanes_per_run = pl.S;
+        out->waves = waves;
+        out->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+        return SMC_OK;
+    });
+}
+
+int smc_run_noalloc(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin, uint64_t n_runs,
+                    const smc_launch *launch, smc_result *out) {
+    if (!m || !q || !out) return fail(SMC_E_INVALID, "null argument");
+    return guarded([&]() -> int {
+        auto wall0 = std::chrono::steady_clock::now();
+        memset(out, 0, sizeof *out);
+        validate_query(q, m->img.n_vars);
+        if (n_runs == 0) throw std::runtime_error("n_runs must be positive");
+        if (n_runs > SMC_NOALLOC_MAX_RUNS)
+            throw std::runtime_error("limit: smc_run_noalloc takes at most 1048576 runs per call (its scratch is fixed)");
+        if (run_begin > (~0ull) - n_runs) throw std::runtime_error("run range overflows");
+        const smc_launch *use =
+            (launch && (launch->lanes_per_run || launch->waves || launch->waves_per_cu || launch->occupancy ||
+                        launch->tables))
+                ? launch : (m->has_tuned ? &m->tuned : nullptr);
+        hip_check(hipSetDevice(m->device), "hipSetDevice");
+        void *scr = nullptr;
+        hip_check(smc::noalloc_scratch(m->device, &scr), "scratch address");
+        uint8_t *base = (uint8_t *)scr;
+        Plan pl = plan_launch(m->img.n_leaves, m->img.n_vars, use ? use->lanes_per_run : 0, use ? use->occupancy : 0,
+                              use ? use->tables : 0, m->table_bytes, use ? use->waves : 0,
+                              use ? use->waves_per_cu : 0, n_runs, n_runs, false, m->cu_count);
+        pl.total_chunks = (n_runs + pl.chunk_runs - 1) / pl.chunk_runs;
+        int waves = pl.waves;
+        if ((uint64_t)waves > pl.total_chunks)
+            waves = (int)((pl.total_chunks + pl.group_waves - 1) / pl.group_waves * pl.group_waves);
+
+        // the per-run buffers are the scratch's (the whole run range of
+        // this call is written by the run kernel before the reduction reads)
+        RunBuffers b;
+        b.flag = base + smc::kNoallocFlag;
+        b.value = (double *)(base + smc::kNoallocValue);
+        b.steps = (uint32_t *)(base + smc::kNoallocSteps);
+        b.trans = (uint32_t *)(base + smc::kNoallocTrans);
+        InstDesc *d_inst = (InstDesc *)(base + smc::kNoallocInst);
+        unsigned long long *d_counter = (unsigned long long *)(base + smc::kNoallocCounter);
+        ReducePartial *d_acc = (ReducePartial *)(base + smc::kNoallocAcc);
+        ReducePartial *d_partials = (ReducePartial *)(base + smc::kNoallocPartials);
+        const uint32_t blocks = reduce_blocks(n_runs);
+        if (blocks > smc::kNoallocPartialsMax) throw std::runtime_error("internal: reduction blocks");
+
+        m->h_inst.m = m->dm;
+        m->h_inst.p = make_params(q, m->img, seed, run_begin, n_runs, b, false, false);
+        m->h_inst.chunk_first = 0;
+        LaunchParams lp{};
+        lp.inst = d_inst;
+        lp.n_inst = 1;
+        lp.lanes = pl.S;
+        lp.var_stride = pl.stride;
+        lp.chunk_runs = pl.chunk_runs;
+        lp.total_chunks = pl.total_chunks;
+        lp.counter = d_counter;
+        lp.round = nullptr;
+        lp.tables = pl.tab ? (decltype(lp.tables))m->arena : nullptr;
+        lp.table_bytes = pl.table_bytes;
+        lp.wave_lds = pl.wave_lds;
+
+        hip_check(hipMemcpyAsync(d_inst, &m->h_inst, sizeof(InstDesc), hipMemcpyHostToDevice, m->stream), "upload instance");
+        hip_check(smc::launch_prep(d_counter, nullptr, 0, m->stream), "prep kernel launch");
+        hip_check(hipEventRecord(m->ev[0], m->stream), "event");
+        hip_check(smc::launch_sim(lp, pl.C, false, pl.occ, waves, pl.group_waves, pl.lds, m->stream), "run kernel launch");
+        hip_check(hipEventRecord(m->ev[1], m->stream), "event");
+        hip_check(smc::launch_reduce(d_inst, 1, blocks, d_partials, d_acc, 0, m->stream), "reduce kernel launch");
+        hip_check(hipEventRecord(m->ev[2], m->stream), "event");
+        hip_check(hipMemcpyAsync(&m->h_acc, d_acc, sizeof(ReducePartial), hipMemcpyDeviceToHost, m->stream), "copy totals");
         hip_check(hipStreamSynchronize(m->stream), "run");
         float k_ms = 0.f, r_ms = 0.f;
         hip_check(hipEventElapsedTime(&k_ms, m->ev[0], m->ev[1]), "event time");

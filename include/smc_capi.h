@@ -124,6 +124,22 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed,
             smc_run_outputs *outputs, smc_result *out);
 
 /*
+ * smc_run without device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate (code under stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same runs, kernels and
+ * totals as smc_run with outputs == NULL, but every intermediate buffer (the
+ * per-run results the reduction reads, the reduction's partials, the work
+ * counter, the instance record) lives in a fixed scratch array of about
+ * 17 MiB per GPU, held until the process ends.  n_runs is at most 2^20
+ * (SMC_E_LIMIT beyond).  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (from several threads or models) race.
+ * Not a graph-capture entry point: it synchronises the model's stream.
+ */
+int smc_run_noalloc(smc_model *m, const smc_query *q, uint64_t seed,
+                    uint64_t run_begin, uint64_t n_runs, const smc_launch *launch,
+                    smc_result *out);
+
+/*
  * Autotuner: time a sample of `sample_runs` runs of `q` with each candidate
  * launch shape (lanes per run from next_pow2(components) down to a
  * sixteenth of it, i.e. 1-16 components per lane, occupancy caps, and
