@@ -5,7 +5,10 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -312,12 +315,12 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
         m->h_inst.m = m->dm;
         m->h_inst.p = make_params(q, m->img, seed, run_begin, n_runs, m->runs, outputs && outputs->time, trace);
         m->h_inst.chunk_first = 0;
+        m->h_inst.lanes = pl.S;
+        m->h_inst.chunk_runs = pl.chunk_runs;
         LaunchParams lp{};
         lp.inst = m->d_inst;
         lp.n_inst = 1;
-        lp.lanes = pl.S;
         lp.var_stride = pl.stride;
-        lp.chunk_runs = pl.chunk_runs;
         lp.total_chunks = pl.total_chunks;
         lp.counter = m->d_counter;
         lp.round = nullptr;
@@ -401,12 +404,12 @@ int smc_run_noalloc(smc_model *m, const smc_query *q, uint64_t seed, uint64_t ru
         m->h_inst.m = m->dm;
         m->h_inst.p = make_params(q, m->img, seed, run_begin, n_runs, b, false, false);
         m->h_inst.chunk_first = 0;
+        m->h_inst.lanes = pl.S;
+        m->h_inst.chunk_runs = pl.chunk_runs;
         LaunchParams lp{};
         lp.inst = d_inst;
         lp.n_inst = 1;
-        lp.lanes = pl.S;
         lp.var_stride = pl.stride;
-        lp.chunk_runs = pl.chunk_runs;
         lp.total_chunks = pl.total_chunks;
         lp.counter = d_counter;
         lp.round = nullptr;
@@ -436,16 +439,119 @@ int smc_run_noalloc(smc_model *m, const smc_query *q, uint64_t seed, uint64_t ru
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Persisted autotuner choices: one line per (model, query, device, library
+// build, sample size) key in a text file — SMC_TUNE_CACHE names it (default
+// $XDG_CACHE_HOME/cusmc_amd/tune.txt or ~/.cache/cusmc_amd/tune.txt;
+// SMC_TUNE_CACHE=off disables it).
+uint64_t fnv1a(uint64_t h, const void *p, size_t n) {
+    const uint8_t *b = (const uint8_t *)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+template <class T>
+uint64_t fnv_vec(uint64_t h, const std::vector<T> &v) {
+    return fnv1a(h, v.data(), v.size() * sizeof(T));
+}
+
+std::string tune_cache_path() {
+    const char *e = std::getenv("SMC_TUNE_CACHE");
+    if (e && *e) return (std::string(e) == "off" || std::string(e) == "0") ? std::string() : std::string(e);
+    const char *x = std::getenv("XDG_CACHE_HOME");
+    const char *home = std::getenv("HOME");
+    if (x && *x) return std::string(x) + "/cusmc_amd/tune.txt";
+    if (home && *home) return std::string(home) + "/.cache/cusmc_amd/tune.txt";
+    return std::string();
+}
+
+uint64_t tune_key(const smc_model *m, const smc_query *q, uint64_t sample_runs) {
+    const smc::ModelImage &im = m->img;
+    uint64_t h = 1469598103934665603ull;
+    h = fnv_vec(h, im.locs); h = fnv_vec(h, im.invs); h = fnv_vec(h, im.edges); h = fnv_vec(h, im.guards);
+    h = fnv_vec(h, im.upds); h = fnv_vec(h, im.consts); h = fnv_vec(h, im.code); h = fnv_vec(h, im.var_init);
+    h = fnv_vec(h, im.comp_init_loc);
+    const int32_t qi[4] = {q->kind, q->var, q->is_max, q->is_estimate};
+    const double qd[2] = {q->threshold, q->time_bound};
+    h = fnv1a(h, qi, sizeof qi); h = fnv1a(h, qd, sizeof qd);
+    h = fnv1a(h, &sample_runs, sizeof sample_runs);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, m->device) == hipSuccess) {
+        h = fnv1a(h, prop.gcnArchName, strnlen(prop.gcnArchName, sizeof prop.gcnArchName));
+        h = fnv1a(h, &prop.multiProcessorCount, sizeof prop.multiProcessorCount);
+    }
+    static const char build[] = __DATE__ " " __TIME__;  // kernels of another build may rank differently
+    return fnv1a(h, build, sizeof build);
+}
+
+bool tune_lookup(const std::string &path, uint64_t key, smc_launch *out) {
+    if (path.empty()) return false;
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return false;
+    unsigned long long k;
+    int S, wpc, occ, tab;
+    bool found = false;
+    while (std::fscanf(f, "%llx %d %d %d %d", &k, &S, &wpc, &occ, &tab) == 5)
+        if (k == key) { *out = smc_launch{S, 0, wpc, occ, tab}; found = true; }
+    std::fclose(f);
+    return found;
+}
+
+void tune_store(const std::string &path, uint64_t key, const smc_launch &c) {
+    if (path.empty()) return;
+    const size_t slash = path.rfind('/');
+    if (slash != std::string::npos) {
+        std::error_code ec;
+        std::filesystem::create_directories(path.substr(0, slash), ec);
+    }
+    if (FILE *f = std::fopen(path.c_str(), "a")) {
+        std::fprintf(f, "%016llx %d %d %d %d\n", (unsigned long long)key, c.lanes_per_run, c.waves_per_cu, c.occupancy,
+                     c.tables);
+        std::fclose(f);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
 int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sample_runs, smc_launch *best) {
     if (!m || !q) return fail(SMC_E_INVALID, "null argument");
     return guarded([&]() -> int {
         if (sample_runs == 0) sample_runs = 65536;
+        validate_query(q, m->img.n_vars);
+        const std::string cache = tune_cache_path();
+        const uint64_t key = tune_key(m, q, sample_runs);
+        smc_launch cached{};
+        if (tune_lookup(cache, key, &cached)) {
+            m->tuned = cached;
+            m->has_tuned = true;
+            if (best) *best = cached;
+            return SMC_OK;
+        }
         const int P = m->img.n_leaves;
+        // lanes per run x register policy x table placement x waves per CU
+        // (all the occupancy allows, or half of it: fewer waves contend less
+        // for the caches and LDS)
         std::vector<smc_launch> cands;
         for (int S = P < 64 ? P : 64; S >= 1; S >>= 1) {
             if (P / S > 16) break;
             for (int occ : {0, 2})
-                for (int tab : {0, 1}) cands.push_back(smc_launch{S, 0, 0, occ, tab});
+                for (int tab : {0, 1}) {
+                    cands.push_back(smc_launch{S, 0, 0, occ, tab});
+                    int blocks = 0;
+                    try {
+                        Plan pl = plan_launch(P, m->img.n_vars, S, occ, tab, m->table_bytes, 0, 0, sample_runs,
+                                              sample_runs, false, m->cu_count);
+                        hip_check(smc::sim_occupancy(pl.C, false, pl.occ, pl.tab, pl.group_waves, pl.lds, &blocks),
+                                  "occupancy query");
+                        const int per_cu = blocks * pl.group_waves;
+                        if (per_cu >= 2 * pl.group_waves) cands.push_back(smc_launch{S, 0, per_cu / 2, occ, tab});
+                    } catch (const std::exception &) {
+                    }
+                }
         }
         // each candidate runs at least twice the runs its grid holds at once,
         // so the sample measures the steady state rather than the tail
@@ -454,8 +560,8 @@ int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sampl
         for (const smc_launch &c : cands) {
             uint64_t n = sample_runs;
             try {
-                Plan pl = plan_launch(P, m->img.n_vars, c.lanes_per_run, c.occupancy, c.tables, m->table_bytes, 0, 0,
-                                      sample_runs, sample_runs, false, m->cu_count);
+                Plan pl = plan_launch(P, m->img.n_vars, c.lanes_per_run, c.occupancy, c.tables, m->table_bytes, 0,
+                                      c.waves_per_cu, sample_runs, sample_runs, false, m->cu_count);
                 n = std::max<uint64_t>(n, 2ull * (uint64_t)pl.waves * (uint64_t)pl.R);
             } catch (const std::exception &) {
                 continue;  // e.g. LDS budget exceeded at this packing
@@ -471,6 +577,7 @@ int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sampl
         m->tuned = pick;
         m->has_tuned = true;
         if (best) *best = pick;
+        tune_store(cache, key, pick);
         return SMC_OK;
     });
 }
@@ -482,23 +589,17 @@ int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sampl
 struct smc_batch {
     int device = 0;
     hipStream_t stream = nullptr;
-    std::vector<hipStream_t> gstreams;
-    std::vector<hipEvent_t> joins;
-    hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<smc_batch_item> items;
     std::vector<RunBuffers> bufs;
-    struct Group {
-        Plan pl;
-        std::vector<int> members;
-        InstDesc *d_inst = nullptr;
-        unsigned long long *d_counter = nullptr;
-        LaunchParams lp{};
-    };
-    std::vector<Group> groups;
+    int C = 8, occ = 0, waves = 1, stride = 1;  // the one launch: slots per lane, register policy
+    size_t lds = 0;
     InstDesc *d_all = nullptr;
+    unsigned long long *d_counter = nullptr;
     ReducePartial *d_partials = nullptr, *d_acc = nullptr;
     uint64_t *d_round = nullptr;
     uint32_t blocks = 1;
+    LaunchParams lp{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
@@ -507,14 +608,70 @@ struct smc_batch {
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
         for (auto &b : bufs) b.release();
-        for (auto &g : groups) { if (g.d_inst) hipFree(g.d_inst); if (g.d_counter) hipFree(g.d_counter); }
-        for (void *p : {(void *)d_all, (void *)d_partials, (void *)d_acc, (void *)d_round}) if (p) hipFree(p);
-        for (auto e : joins) hipEventDestroy(e);
-        for (hipEvent_t e : {fork, ev0, ev1}) if (e) hipEventDestroy(e);
-        for (auto s : gstreams) hipStreamDestroy(s);
+        for (void *p : {(void *)d_all, (void *)d_counter, (void *)d_partials, (void *)d_acc, (void *)d_round})
+            if (p) hipFree(p);
+        for (hipEvent_t e : {ev0, ev1}) if (e) hipEventDestroy(e);
         if (stream) hipStreamDestroy(stream);
     }
 };
+
+namespace {
+
+// The instance records of one batch launch with C slots per lane: lanes per
+// run S_i = next_pow2(components_i) / C (at least 1), chunks of k R_i runs
+// (R_i = 64 / S_i runs per wave) sized so the queue holds several chunks per
+// wave; returns the LDS bytes one wave needs (the largest instance) and the
+// total chunk count.  `runs` overrides every instance's n_runs (tuning
+// samples).
+struct BatchShape {
+    std::vector<InstDesc> inst;
+    size_t wave_lds = 0;
+    uint64_t chunks = 0;
+};
+
+BatchShape batch_shape(smc_batch *b, int C, int waves, uint64_t runs_override) {
+    BatchShape sh;
+    uint64_t total = 0;
+    for (auto &it : b->items) total += runs_override ? std::min<uint64_t>(runs_override, it.n_runs) : it.n_runs;
+    const uint64_t per_wave = (total + (uint64_t)waves - 1) / (uint64_t)waves;
+    for (size_t i = 0; i < b->items.size(); ++i) {
+        const smc_batch_item &it = b->items[i];
+        const int P = it.model->img.n_leaves;
+        if (P > 64 * C) throw std::runtime_error("internal: instance wider than the launch's packing");
+        const int S = P / C >= 1 ? P / C : 1;
+        const int R = 64 / S;
+        int levels = 0;
+        while ((1 << levels) < S) ++levels;
+        const size_t wave = ((size_t)R * b->stride * 8 + (size_t)2 * levels * 64 * 8 + 15) & ~(size_t)15;
+        sh.wave_lds = std::max(sh.wave_lds, wave);
+        const uint64_t n = runs_override ? std::min<uint64_t>(runs_override, it.n_runs) : it.n_runs;
+        uint64_t k = (per_wave + 4ull * R - 1) / (4ull * R);
+        k = std::max<uint64_t>(1, std::min<uint64_t>(k, 64));
+        const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(k * R, n));
+        InstDesc d{};
+        d.m = it.model->dm;
+        d.p = make_params(&it.query, it.model->img, it.seed, it.run_begin, n, b->bufs[i], false, false);
+        d.chunk_first = sh.chunks;
+        d.lanes = S;
+        d.chunk_runs = (uint32_t)chunk;
+        sh.chunks += (n + chunk - 1) / chunk;
+        sh.inst.push_back(d);
+    }
+    if (sh.wave_lds > 60000) throw std::runtime_error("limit: variables of the packed runs exceed the LDS budget");
+    return sh;
+}
+
+// Waves of one launch at C slots per lane and register policy occ: every
+// CU filled to its occupancy (or to waves_per_cu when that is lower).
+int batch_waves(int C, int occ, size_t wave_lds, int waves_per_cu, int cu_count) {
+    int blocks = 0;
+    hip_check(smc::sim_occupancy(C, false, occ, false, 1, wave_lds, &blocks), "occupancy query");
+    int per_cu = blocks;
+    if (waves_per_cu > 0 && waves_per_cu < per_cu) per_cu = waves_per_cu;
+    return std::max(1, per_cu) * cu_count;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -525,126 +682,99 @@ int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_bat
         auto b = std::make_unique<smc_batch>();
         b->items.assign(items, items + n_items);
         b->device = items[0].model ? items[0].model->device : 0;
+        int maxP = 1, max_vars = 1;
         for (int i = 0; i < n_items; ++i) {
             const smc_batch_item &it = items[i];
             if (!it.model) throw std::runtime_error("batch item without a model");
             if (it.model->device != b->device) throw std::runtime_error("batch items on different devices");
             validate_query(&it.query, it.model->img.n_vars);
             if (it.n_runs == 0 || it.n_runs > (1ull << 32) - 1) throw std::runtime_error("batch item n_runs out of range");
+            maxP = std::max(maxP, it.model->img.n_leaves);
+            max_vars = std::max(max_vars, it.model->img.n_vars);
         }
         hip_check(hipSetDevice(b->device), "hipSetDevice");
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
-        for (hipEvent_t *e : {&b->fork, &b->ev0, &b->ev1}) hip_check(hipEventCreate(e), "hipEventCreate");
+        for (hipEvent_t *e : {&b->ev0, &b->ev1}) hip_check(hipEventCreate(e), "hipEventCreate");
         const int cu = items[0].model->cu_count;
-
-        // group by launch shape: tuned (per size class) or default
-        std::map<std::tuple<int, int, int>, std::vector<int>> by_shape;  // (P, S, occupancy) -> members
-        std::map<int, smc_launch> tuned;                            // P -> lanes, occupancy
-        if (tune) {
-            std::map<int, int> rep;  // P -> item with the most components... first seen
-            for (int i = 0; i < n_items; ++i) {
-                const int P = items[i].model->img.n_leaves;
-                auto it = rep.find(P);
-                if (it == rep.end() || items[i].model->img.n_comp > items[it->second].model->img.n_comp) rep[P] = i;
-            }
-            for (auto &kv : rep) {
-                const smc_batch_item &it = items[kv.second];
-                smc_launch best{};
-                uint64_t sample = std::min<uint64_t>(it.n_runs, 16384);
-                if (smc_autotune(it.model, &it.query, it.seed, sample, &best) != SMC_OK)
-                    throw std::runtime_error(std::string("autotune failed: ") + g_last_error);
-                tuned[kv.first] = best;
-            }
-        }
-        for (int i = 0; i < n_items; ++i) {
-            const smc_model *m = items[i].model;
-            const int P = m->img.n_leaves;
-            const smc_launch *t = tuned.count(P) ? &tuned[P] : (m->has_tuned ? &m->tuned : nullptr);
-            int S = t ? t->lanes_per_run : 0;
-            if (!S) S = P < 64 ? P : 64;
-            by_shape[{P, S, t ? t->occupancy : 0}].push_back(i);
-        }
+        b->stride = max_vars | 1;
         b->bufs.resize(n_items);
-        std::vector<InstDesc> all(n_items);
-        for (auto &kv : by_shape) {
-            smc_batch::Group g;
-            g.members = kv.second;
-            uint64_t total = 0, maxr = 0;
-            int max_vars = 1;
-            for (int i : g.members) {
-                total += items[i].n_runs;
-                maxr = std::max<uint64_t>(maxr, items[i].n_runs);
-                max_vars = std::max(max_vars, items[i].model->img.n_vars);
-            }
-            g.pl = plan_launch(std::get<0>(kv.first), max_vars, std::get<1>(kv.first), std::get<2>(kv.first), 0, 0, 0,
-                               0, total, maxr, false, cu);
-            std::vector<InstDesc> inst;
-            uint64_t chunk = 0;
-            for (int i : g.members) {
-                const smc_batch_item &it = items[i];
-                b->bufs[i].ensure(it.n_runs);
-                InstDesc d{};
-                d.m = it.model->dm;
-                d.p = make_params(&it.query, it.model->img, it.seed, it.run_begin, it.n_runs, b->bufs[i], false, false);
-                d.chunk_first = chunk;
-                chunk += (it.n_runs + g.pl.chunk_runs - 1) / g.pl.chunk_runs;
-                inst.push_back(d);
-                all[i] = d;
-            }
-            g.pl.total_chunks = chunk;
-            if ((uint64_t)g.pl.waves > chunk) g.pl.waves = (int)chunk;
-            hip_check(hipMalloc(&g.d_inst, inst.size() * sizeof(InstDesc)), "hipMalloc group");
-            hip_check(hipMemcpy(g.d_inst, inst.data(), inst.size() * sizeof(InstDesc), hipMemcpyHostToDevice), "upload group");
-            hip_check(hipMalloc(&g.d_counter, sizeof(unsigned long long)), "hipMalloc counter");
-            hip_check(hipMemset(g.d_counter, 0, sizeof(unsigned long long)), "clear counter");
-            b->groups.push_back(g);
-        }
         uint64_t maxr = 0;
-        for (auto &it : b->items) maxr = std::max<uint64_t>(maxr, it.n_runs);
+        for (int i = 0; i < n_items; ++i) {
+            b->bufs[i].ensure(items[i].n_runs);
+            maxr = std::max<uint64_t>(maxr, items[i].n_runs);
+        }
         b->blocks = std::min<uint32_t>(reduce_blocks(maxr), 64);
         hip_check(hipMalloc(&b->d_all, n_items * sizeof(InstDesc)), "hipMalloc instances");
-        hip_check(hipMemcpy(b->d_all, all.data(), n_items * sizeof(InstDesc), hipMemcpyHostToDevice), "upload instances");
+        hip_check(hipMalloc(&b->d_counter, sizeof(unsigned long long)), "hipMalloc counter");
         hip_check(hipMalloc(&b->d_partials, (size_t)n_items * b->blocks * sizeof(ReducePartial)), "hipMalloc partials");
         hip_check(hipMalloc(&b->d_acc, n_items * sizeof(ReducePartial)), "hipMalloc totals");
         hip_check(hipMalloc(&b->d_round, sizeof(uint64_t)), "hipMalloc round");
         hip_check(hipMemset(b->d_round, 0, sizeof(uint64_t)), "clear round");
-        for (auto &g : b->groups) {
-            g.lp.inst = g.d_inst;
-            g.lp.n_inst = (int32_t)g.members.size();
-            g.lp.lanes = g.pl.S;
-            g.lp.var_stride = g.pl.stride;
-            g.lp.chunk_runs = g.pl.chunk_runs;
-            g.lp.total_chunks = g.pl.total_chunks;
-            g.lp.counter = g.d_counter;
-            g.lp.round = b->d_round;
-            g.lp.tables = nullptr;
-            g.lp.table_bytes = 0;
-            g.lp.wave_lds = g.pl.wave_lds;
+
+        // the launch's slots per lane: the smallest that holds the widest
+        // instance in 64 lanes, or (tune) the fastest of that and wider
+        // packings under both register policies on a sample of every
+        // instance's runs
+        int Cmin = 1;
+        while (64 * Cmin < maxP) Cmin <<= 1;
+        if (Cmin > 16) throw std::runtime_error("limit: more than 1024 components per network");
+        auto set_launch = [&](const BatchShape &sh, int C, int occ, int waves) {
+            b->C = C;
+            b->occ = occ;
+            b->waves = (int)std::min<uint64_t>((uint64_t)waves, std::max<uint64_t>(sh.chunks, 1));
+            b->lds = sh.wave_lds;
+            b->lp = LaunchParams{};
+            b->lp.inst = b->d_all;
+            b->lp.n_inst = n_items;
+            b->lp.var_stride = b->stride;
+            b->lp.total_chunks = sh.chunks;
+            b->lp.counter = b->d_counter;
+            b->lp.round = b->d_round;
+            b->lp.tables = nullptr;
+            b->lp.table_bytes = 0;
+            b->lp.wave_lds = (uint32_t)sh.wave_lds;
+            hip_check(hipMemcpy(b->d_all, sh.inst.data(), n_items * sizeof(InstDesc), hipMemcpyHostToDevice),
+                      "upload instances");
+        };
+        int bestC = std::max(Cmin, 8), best_occ = 0;
+        if (tune) {
+            double best_ms = 1e300;
+            uint64_t sample = 0;
+            for (auto &it : b->items) sample = std::max<uint64_t>(sample, std::min<uint64_t>(it.n_runs, 2048));
+            for (int C = Cmin; C <= 16; C <<= 1)
+                for (int occ : {0, 2}) {
+                    BatchShape probe = batch_shape(b.get(), C, 1, sample);
+                    const int waves = batch_waves(C, occ, probe.wave_lds, 0, cu);
+                    BatchShape sh = batch_shape(b.get(), C, waves, sample);
+                    set_launch(sh, C, occ, waves);
+                    float ms = 0.f;
+                    for (int rep = 0; rep < 2; ++rep) {  // the first launch warms the code
+                        hip_check(smc::launch_prep(b->d_counter, nullptr, 0, b->stream), "prep");
+                        hip_check(hipEventRecord(b->ev0, b->stream), "event");
+                        hip_check(smc::launch_sim(b->lp, C, false, occ, b->waves, 1, b->lds, b->stream), "tune launch");
+                        hip_check(hipEventRecord(b->ev1, b->stream), "event");
+                        hip_check(hipStreamSynchronize(b->stream), "tune");
+                        hip_check(hipEventElapsedTime(&ms, b->ev0, b->ev1), "event time");
+                    }
+                    if (ms < best_ms) { best_ms = ms; bestC = C; best_occ = occ; }
+                }
         }
-        b->gstreams.resize(b->groups.size());
-        b->joins.resize(b->groups.size());
-        for (size_t k = 0; k < b->groups.size(); ++k) {
-            hip_check(hipStreamCreateWithFlags(&b->gstreams[k], hipStreamNonBlocking), "hipStreamCreate");
-            hip_check(hipEventCreateWithFlags(&b->joins[k], hipEventDisableTiming), "hipEventCreate");
+        {
+            BatchShape probe = batch_shape(b.get(), bestC, 1, 0);
+            const int waves = batch_waves(bestC, best_occ, probe.wave_lds, 0, cu);
+            BatchShape sh = batch_shape(b.get(), bestC, waves, 0);
+            set_launch(sh, bestC, best_occ, waves);
         }
         hip_check(hipDeviceSynchronize(), "batch setup");
 
-        // capture one round: groups run concurrently, then the reduction
-        // accumulates, then the counters are cleared and the round advances
+        // capture one round: the run kernel over every instance, the
+        // reduction accumulates, then the counter is cleared and the round
+        // advances
         hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-        hip_check(hipEventRecord(b->fork, b->stream), "fork");
-        for (size_t k = 0; k < b->groups.size(); ++k) {
-            auto &g = b->groups[k];
-            hip_check(hipStreamWaitEvent(b->gstreams[k], b->fork, 0), "fork wait");
-            hip_check(smc::launch_sim(g.lp, g.pl.C, false, g.pl.occ, g.pl.waves, g.pl.group_waves, g.pl.lds, b->gstreams[k]),
-                      "capture run kernel");
-            hip_check(hipEventRecord(b->joins[k], b->gstreams[k]), "join");
-            hip_check(hipStreamWaitEvent(b->stream, b->joins[k], 0), "join wait");
-        }
+        hip_check(smc::launch_prep(b->d_counter, nullptr, 0, b->stream), "capture prep");
+        hip_check(smc::launch_sim(b->lp, b->C, false, b->occ, b->waves, 1, b->lds, b->stream), "capture run kernel");
         hip_check(smc::launch_reduce(b->d_all, n_items, b->blocks, b->d_partials, b->d_acc, 1, b->stream), "capture reduce");
-        for (size_t k = 0; k < b->groups.size(); ++k)
-            hip_check(smc::launch_prep(b->groups[k].d_counter, k == 0 ? b->d_round : nullptr, k == 0 ? 1 : 0, b->stream),
-                      "capture prep");
+        hip_check(smc::launch_prep(b->d_counter, b->d_round, 1, b->stream), "capture advance");
         hip_check(hipStreamEndCapture(b->stream, &b->graph), "end capture");
         hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "instantiate graph");
         *out = b.release();
@@ -685,7 +815,7 @@ int smc_batch_run(smc_batch *b, int rounds, smc_result *results, smc_batch_stats
             stats->runs = runs;
             stats->steps = steps;
             stats->transitions = trans;
-            stats->groups = (int32_t)b->groups.size();
+            stats->groups = 1;
             stats->rounds = rounds;
         }
         return SMC_OK;

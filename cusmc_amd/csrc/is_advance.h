@@ -17,18 +17,22 @@
 // Cross-entropy training replays the elite runs once with the tilts that
 // generated them and records, per (run, macro-step), the features at the
 // macro-step's start and two statistics per location L of the run's
-// components: A_L, the number of races won by the component in L, and
-// B_L = ln2 * dt * r_L, its base rate times the macro-step's duration.  The
-// log-density of the macro-step under tilts theta is then
-//   ln q = sum_L ln2 (A_L theta_L - B_L 2^theta_L) + terms free of theta,
+// components: A_L, the number of races won by the component in L, and B_L,
+// its base cumulative hazard over the macro-step's races (r ln2 d per race
+// for an exponential racer, -ln(1 - d/M) for a uniform racer on the window
+// [0, M]).  The log-density of the macro-step under tilts theta is then
+//   ln q = sum_L (ln2 theta_L A_L - 2^theta_L B_L) + terms free of theta,
 // so every training epoch is a pass of GEMMs over the recorded pairs.
 //
-// Likelihood ratio of one race step (exponential race with base rates r_c,
-// tilted rates r'_c = r_c 2^theta_c, sums R and R', delay d, delays drawn
-// as -log2(u)/rate so the effective rates carry a factor ln 2):
-//   LR = (r_w / r'_w) exp(-(R - R') ln2 d)   =>   log2 LR = -theta_w + (R' - R) d
-// where the first term is present only when an exponential component won.
-// Uniform and zero delays are not tilted and contribute nothing.
+// The proposal: an exponential component's rate r becomes r' = r 2^theta;
+// a uniform (clock-bounded) component's hazard 1/(M - d) is multiplied by
+// a = 2^theta (a delay-window tilt: survival (1 - d/M)^a on the same window,
+// drawn as M (1 - (1 - u)^(1/a))).  Likelihood ratio of one race step
+// (exponential base rates r_c summing to R, tilted to R'; delays drawn as
+// -log2(u)/rate, so the effective rates carry a factor ln 2):
+//   log2 LR = -theta_w + (R' - R) d + sum_{uniform c} (1 - a_c) log2(1 - d/M_c)
+// where theta_w is present when an exponential or uniform racer won with its
+// own draw.  Zero delays are not tilted and contribute nothing.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -107,7 +111,6 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
     const IsRows &st = P.st;
     const int n = m.n_comp;
     const double INF = __builtin_huge_val();
-    const double LN2 = 0.69314718055994530942;
 
     uint8_t stat = st.status[row];
     if (!(stat & IS_ACTIVE)) return;  // its features and tilts are not read
@@ -115,7 +118,6 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
     // load the run
     const uint64_t run = st.run[row];
     double t = st.t[row];
-    const double t_start = t;
     double log2lr = st.log2lr[row];
     uint32_t step = st.step[row], trans = st.trans[row];
     int chan = st.chan[row], sender = st.sender[row];
@@ -125,9 +127,9 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
     uint64_t digest = st.digest[row];
     for (int v = sl; v < m.n_vars; v += S) vars[v] = st.vars[(size_t)row * m.n_vars + v];
     int loc[C];
-    // recording build: per slot the starting location, the rates at the
-    // macro-step's first race and the exponential wins, in LDS ([slot][lane])
-    double *r0 = tl.rec + lane;
+    // recording build: per slot the starting location, the base cumulative
+    // hazard over the macro-step's races and the wins, in LDS ([slot][lane])
+    double *hz = tl.rec + lane;
     int32_t *loc0 = (int32_t *)(tl.rec + 64 * C) + lane;
     uint32_t *wins = (uint32_t *)(tl.rec + 64 * C) + 64 * C + lane;
 #pragma unroll
@@ -136,7 +138,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
         loc[j] = c < n ? (int)st.loc[(size_t)row * n + c] : -1;
         if (REC) {
             loc0[64 * j] = loc[j];
-            r0[64 * j] = 0.0;
+            hz[64 * j] = 0.0;
             wins[64 * j] = 0;
         }
     }
@@ -167,7 +169,6 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
         write_features<C>(P, P.rec_X + (size_t)rec * P.fpad, loc, vars, t, sl, S, n);
         if (sl == 0) P.rec_w[rec] = P.weight[st.job[row]];
     }
-    bool first_race = true;
     bool hit = false, end = false, fired_any = false;
     bool changed = true;  // state changed since the last absorption check
     uint32_t blk = 0;     // blocked slots at the last change of state
@@ -281,25 +282,24 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                         slot_set<C>(leaft, j, lt);
                     }
                 }
-                if (first_race) {
-                    if (REC) {
-#pragma unroll
-                        for (int j = 0; j < C; ++j) r0[64 * j] = leaf[j];
-                    }
-                    first_race = false;
-                }
                 const uint64_t zb = __ballot(zero_j < C) & seg_mask;
                 if (zb) {
                     const int Lz = lowest_lane(zb);
                     w = (Lz - seg_base) * C + __shfl(zero_j, Lz);
                     d = 0.0;
                 } else {
+                    // uniform delays; a tilted window (hazard times a = 2^theta)
+                    // draws M (1 - (1 - u)^(1/a))
                     double umin = INF;
                     int unif_j = -1;
                     for (uint32_t b = bmask; b; b &= b - 1) {
                         const int j = __ffs(b) - 1;
                         const smc_u32x4 rw = smc_draw(P.seed, run, step, SMC_P_UNIFORM, (uint32_t)(sl * C + j));
-                        const double du = slot_get<C>(Mb, j) * smc_u01(rw.v[0], rw.v[1]);
+                        const double u = smc_u01(rw.v[0], rw.v[1]);
+                        const double M = slot_get<C>(Mb, j);
+                        const int ti = theta ? tl.tix[slot_get<C>(loc, j)] : -1;
+                        const bool tilted = ti >= 0 && theta[ti] != 0.0f;
+                        const double du = tilted ? M * (1.0 - smc_exp2(smc_log2(1.0 - u) / tex[ti])) : M * u;
                         if (du < umin) { umin = du; unif_j = j; }
                     }
                     // tilted race: the run kernel's tree over the tilted rates
@@ -356,7 +356,8 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                         const int Lu = lowest_lane(__ballot(unif_j >= 0 && umin == gu) & seg_mask);
                         wu = (Lu - seg_base) * C + __shfl(unif_j, Lu);
                     }
-                    if (gu < dexp || (gu == dexp && wu < wexp)) { d = gu; w = wu; }
+                    bool u_won = false;
+                    if (gu < dexp || (gu == dexp && wu < wexp)) { d = gu; w = wu; u_won = true; }
                     else { d = dexp; w = wexp; exp_won = true; }
                     // the step that crosses the bound: the left-out null
                     // racers' first event after it (base rates, no tilt)
@@ -370,9 +371,11 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                         }
                     }
                     if (theta && d < INF) {
-                        // log2 LR of this step: -theta_w + (R' - R) d
+                        // log2 LR of this step: -theta_w (the winner's own
+                        // draw) + (R' - R) d + the tilted uniform racers'
+                        // (1 - a) log2(1 - d/M), summed on the race tree
                         double th_w = 0.0;
-                        if (exp_won) {
+                        if (exp_won || u_won) {
                             const int wl_lane = seg_base + w / C;
                             const int wj = w % C;
                             double tw = 0.0;
@@ -384,10 +387,32 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                             th_w = __shfl(tw, wl_lane);
                         }
                         log2lr = log2lr - th_w + (Rt - R) * d;
+                        if (__ballot(bmask != 0) & seg_mask) {
+                            double ut[C];
+#pragma unroll
+                            for (int j = 0; j < C; ++j) {
+                                ut[j] = 0.0;
+                                if ((bmask >> j) & 1u) {
+                                    const int ti = tl.tix[loc[j]];
+                                    if (ti >= 0 && theta[ti] != 0.0f)
+                                        ut[j] = (1.0 - tex[ti]) * smc_log2(1.0 - d / Mb[j]);
+                                }
+                            }
+                            log2lr = log2lr + tree_total<C>(ut, S);
+                        }
                     }
-                    if (exp_won && rec >= 0) {
+                    if ((exp_won || u_won) && rec >= 0) {
                         const int wj = w % C;
                         if (lane == seg_base + w / C) wins[64 * wj] += 1u;
+                    }
+                    if (REC && rec >= 0 && d < INF) {
+                        // each racer's base cumulative hazard over this step
+                        const double LN2 = 0.69314718055994530942;
+#pragma unroll
+                        for (int j = 0; j < C; ++j) {
+                            if (leaf[j] > 0.0) hz[64 * j] = hz[64 * j] + LN2 * leaf[j] * d;
+                            else if ((bmask >> j) & 1u) hz[64 * j] = hz[64 * j] + -LN2 * smc_log2(1.0 - d / Mb[j]);
+                        }
                     }
                 }
                 if (!(d < INF)) { t = INF; end = true; break; }
@@ -494,10 +519,10 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
     if (P.feat) write_features<C>(P, P.feat + (size_t)row * P.fpad, loc, vars, t, sl, S, n);
     if (rec >= 0) {
         // statistics of the macro-step for the locations its components
-        // started in (rates are constant within a macro-step)
+        // started in (a component changes location only at the macro-step's
+        // end): wins and base cumulative hazard
         float *A = P.rec_A + (size_t)rec * P.lpad;
         float *B = P.rec_B + (size_t)rec * P.lpad;
-        const double dt = (t < INF ? t : P.bound) - t_start;
 #pragma unroll
         for (int j = 0; j < C; ++j) {
             const int c = sl * C + j;
@@ -506,7 +531,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     float a = 0.f, b = 0.f;
                     if (l == loc0[64 * j] && P.tiltable[l]) {
                         a = (float)wins[64 * j];
-                        b = (float)(LN2 * dt * r0[64 * j]);
+                        b = (float)hz[64 * j];
                     }
                     A[l] = a;
                     B[l] = b;

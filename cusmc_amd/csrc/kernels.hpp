@@ -25,20 +25,21 @@ struct SimParams {
     double *time;               // [n_runs] optional
 };
 
-// One (model, query, runs) instance of a launch.
+// One (model, query, runs) instance of a launch.  Instances of one launch
+// share the kernel's slots per lane C and may differ in lanes per run
+// (S = next_pow2(components) / C, at least 1).
 struct InstDesc {
     DevModel m;
     SimParams p;
     uint64_t chunk_first;       // first global chunk of this instance
-    uint64_t pad;
+    int32_t lanes;              // S, lanes per run (power of two <= 64)
+    uint32_t chunk_runs;        // runs per chunk of this instance
 };
 
 struct LaunchParams {
     const InstDesc *inst;       // sorted by chunk_first
     int32_t n_inst;
-    int32_t lanes;              // S, lanes per run (power of two <= 64)
     int32_t var_stride;         // doubles of LDS per segment (>= every n_vars)
-    uint32_t chunk_runs;        // runs per chunk
     uint64_t total_chunks;
     unsigned long long *counter;  // chunks handed out (cleared by launch_prep)
     const uint64_t *round;      // optional: runs of round r start at run_begin + r * n_runs

@@ -60,7 +60,8 @@ namespace {
 //
 // LDS: [tables: lp.table_bytes, TAB only] then per wave [variables of the
 // wave's 64/S runs: var_stride doubles each][race-tree levels: 2 log2(S)
-// doubles per lane, level-major].
+// doubles per lane, level-major]; lp.wave_lds covers the largest of these
+// over the launch's instances.
 template <int C, bool TRACE, bool TAB>
 __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
     extern __shared__ __attribute__((aligned(16))) double lds[];
@@ -68,17 +69,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     unsigned int &chunk_next = chunk_next_of[wv];
-    const int S = lp.lanes;
-    const int sl = lane & (S - 1);
-    const int seg_base = lane - sl;
-    const uint64_t seg_mask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << seg_base);
     double *wbase = lds + (TAB ? lp.table_bytes / 8 : 0) + (size_t)wv * (lp.wave_lds / 8);
-    double *vars = wbase + (size_t)(lane / S) * lp.var_stride;
-    // per lane: the race tree's partial sums at each butterfly level (own at
-    // [64 (2k)], sibling block's at [64 (2k + 1)]), kept in LDS rather than
-    // registers from the sum to the descent; level-major, so the 64 lanes
-    // touch 64 consecutive doubles (no bank conflicts)
-    double *lvl = wbase + (size_t)(64 / S) * lp.var_stride + lane;
     const double INF = __builtin_huge_val();
     const uint64_t round = lp.round ? *lp.round : 0;
     using Tables = typename std::conditional<TAB, SmcLdsTables, SmcGlobalTables>::type;
@@ -100,9 +91,21 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
     if constexpr (TAB) m = lds_model(mg, lp.tables, lds);
     else m = mg;
     const SimParams p = lp.inst[lo].p;
+    // the instance's packing: S lanes per run, 64 / S runs per wave
+    const int S = lp.inst[lo].lanes;
+    const int sl = lane & (S - 1);
+    const int seg_base = lane - sl;
+    const uint64_t seg_mask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << seg_base);
+    double *vars = wbase + (size_t)(lane / S) * lp.var_stride;
+    // per lane: the race tree's partial sums at each butterfly level (own at
+    // [64 (2k)], sibling block's at [64 (2k + 1)]), kept in LDS rather than
+    // registers from the sum to the descent; level-major, so the 64 lanes
+    // touch 64 consecutive doubles (no bank conflicts)
+    double *lvl = wbase + (size_t)(64 / S) * lp.var_stride + lane;
+    const uint32_t chunk_runs = lp.inst[lo].chunk_runs;
     const uint64_t chunk = g - lp.inst[lo].chunk_first;
-    const uint32_t r_first = (uint32_t)(chunk * lp.chunk_runs);
-    const uint32_t r_end = (uint32_t)((uint64_t)r_first + lp.chunk_runs < p.n_runs ? r_first + lp.chunk_runs : p.n_runs);
+    const uint32_t r_first = (uint32_t)(chunk * chunk_runs);
+    const uint32_t r_end = (uint32_t)((uint64_t)r_first + chunk_runs < p.n_runs ? r_first + chunk_runs : p.n_runs);
     if (lane == 0) chunk_next = r_first;
     wave_sync();
     const int n = m.n_comp;

@@ -142,11 +142,16 @@ int smc_run_noalloc(smc_model *m, const smc_query *q, uint64_t seed,
 /*
  * Autotuner: time a sample of `sample_runs` runs of `q` with each candidate
  * launch shape (lanes per run from next_pow2(components) down to a
- * sixteenth of it, i.e. 1-16 components per lane, occupancy caps, and
- * tables in global memory or in LDS when they fit) and
- * keep the fastest in the model; smc_run uses it whenever its launch
- * argument is NULL or all zero.  The chosen shape is written to *best.
- * Results of smc_run do not depend on the shape (bit-identical per run).
+ * sixteenth of it, i.e. 1-16 components per lane; register policy; tables
+ * in global memory or in LDS when they fit; waves per CU at the occupancy
+ * limit or half of it) and keep the fastest in the model; smc_run uses it
+ * whenever its launch argument is NULL or all zero.  The chosen shape is
+ * written to *best.  Choices persist in a cache file keyed by the model,
+ * the query, the device and the library build: SMC_TUNE_CACHE names it
+ * (default $XDG_CACHE_HOME/cusmc_amd/tune.txt, else
+ * ~/.cache/cusmc_amd/tune.txt; "off" disables it); a cached key is not
+ * timed again.  Results of smc_run do not depend on the shape
+ * (bit-identical per run).
  */
 int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sample_runs, smc_launch *best);
 
@@ -154,12 +159,14 @@ int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sampl
 
 /*
  * Many (model, query, run range) instances with uneven component counts in
- * one launch: instances are grouped by launch shape, each group is one
- * persistent kernel over a queue of (instance, chunk of runs), the groups
- * run concurrently, and one reduction folds every instance's runs.  The
- * whole round is captured once in a HIP graph; each replay simulates the
- * next n_runs runs of every instance (round r covers run indices
- * run_begin + r*n_runs ...), and totals accumulate across rounds.
+ * one persistent launch: one run kernel over a queue of (instance, chunk of
+ * runs); the launch holds C component slots per lane and each instance
+ * packs a run into S_i = next_pow2(components_i) / C lanes (64 / S_i runs
+ * per wave), so narrow and wide networks share the grid; one reduction
+ * folds every instance's runs.  The whole round is captured once in a HIP
+ * graph; each replay simulates the next n_runs runs of every instance
+ * (round r covers run indices run_begin + r*n_runs ...), and totals
+ * accumulate across rounds.
  * No counterpart in the reference (one model per process, smc.cu:283).
  */
 typedef struct smc_batch smc_batch;
@@ -176,11 +183,14 @@ typedef struct smc_batch_stats {
     double graph_ms;        /* HIP-event time of all replays of the last run  */
     uint64_t runs;          /* runs simulated by the last smc_batch_run       */
     uint64_t steps, transitions;
-    int32_t groups;         /* launch shapes (concurrent kernels) per round   */
+    int32_t groups;         /* run-kernel launches per round (1)              */
     int32_t rounds;
 } smc_batch_stats;
 
-/* tune != 0: autotune one representative instance per group first. */
+/* tune != 0: time one round on a sample of every instance's runs for each
+ * slots-per-lane value (from the smallest that fits the widest instance up
+ * to 16) and register policy, and keep the fastest; else C = 8, or the
+ * smallest that fits. */
 int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_batch **out);
 /* Replay `rounds` rounds; results[i] = totals of item i over these rounds. */
 int smc_batch_run(smc_batch *b, int rounds, smc_result *results, smc_batch_stats *stats);

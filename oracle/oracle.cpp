@@ -193,6 +193,7 @@ public:
         : n_(n), q_(q), seed_(seed), mode_(mode),
           max_steps_(q.max_steps ? q.max_steps : SMC_DEFAULT_MAX_STEPS) {
         leaf_.resize(n.n_components);
+        um_.resize(n.n_components);
         loc_.resize(n.n_components);
         vars_.resize(n.n_vars > 0 ? n.n_vars : 1);
         recv_.resize(n.n_components);
@@ -253,12 +254,12 @@ public:
     RunOut run(uint64_t run, std::vector<int32_t> *events = nullptr, std::vector<double> *times = nullptr,
                const IsOpts *is = nullptr) {
         is_ = is;
+        hazards_ = nullptr;
         log2lr_ = 0.0;
         const bool record = is && is->X;
         bool new_macro = true, open = false;
-        double t_start = 0.0;
         std::vector<int> loc0;
-        std::vector<double> r0;
+        std::vector<double> hz;
         std::vector<uint32_t> wins;
         const int N = n_.n_components;
         for (int v = 0; v < n_.n_vars; ++v) vars_[v] = n_.var_init[v];
@@ -281,10 +282,8 @@ public:
             if (times) times->push_back(t);
         };
 
-        auto close_macro = [&](double t_end) {
+        auto close_macro = [&]() {
             if (!open) return;
-            const double LN2 = 0.69314718055994530942;
-            const double dt = (t_end < HUGE_VAL ? t_end : q_.time_bound) - t_start;
             const size_t base = is->A->size();
             is->A->resize(base + is->lpad, 0.f);
             is->B->resize(base + is->lpad, 0.f);
@@ -292,7 +291,7 @@ public:
                 const int l = loc0[c];
                 if (n_.locations[l].type > SMC_LOC_GOAL) continue;  // not tiltable
                 (*is->A)[base + l] = (float)wins[c];
-                (*is->B)[base + l] = (float)(LN2 * dt * r0[c]);
+                (*is->B)[base + l] = (float)hz[c];
             }
             open = false;
         };
@@ -325,7 +324,7 @@ public:
             }
 
             if (record && new_macro) {
-                close_macro(t);
+                close_macro();
                 // features: one-hot locations, scaled variables, t / bound
                 const size_t base = is->X->size();
                 is->X->resize(base + is->fpad, 0);
@@ -336,11 +335,10 @@ public:
                     f[nl + v] = bf16_rne((float)(vars_[v] * (double)is->var_scale[v]));
                 f[nl + n_.n_vars] = bf16_rne((float)(t / q_.time_bound));
                 loc0.assign(loc_.begin(), loc_.end());
-                r0.assign(N, 0.0);
+                hz.assign(N, 0.0);
                 wins.assign(N, 0);
-                t_start = t;
                 open = true;
-                base_rates_ = &r0;
+                hazards_ = &hz;
             }
             new_macro = false;
             // A goal entered by a broadcast receiver (this iteration's
@@ -365,11 +363,10 @@ public:
 
             double d;
             int w;
-            bool exp_won = false;
+            bool won = false;
             if (mode_ == ORACLE_LITERAL) race_literal(run, step, &d, &w);
-            else race_canonical(run, step, t, goal_pending, &d, &w, &exp_won);
-            base_rates_ = nullptr;
-            if (record && exp_won) ++wins[w];
+            else race_canonical(run, step, t, goal_pending, &d, &w, &won);
+            if (record && won) ++wins[w];
             if (!(d < HUGE_VAL)) { t = HUGE_VAL; break; }
 
             for (int v : clocks_) vars_[v] += d;
@@ -389,7 +386,7 @@ public:
             t = t + d;
             ++step;
         }
-        if (record) close_macro(t);
+        if (record) close_macro();
 
         RunOut o;
         o.value = q_.is_max ? qmax_ : qmin_;
@@ -416,9 +413,10 @@ private:
     std::vector<std::vector<int>> recv_;
     RaceTree tree_, base_tree_, dead_tree_;
     std::vector<double> dleaf_;
+    std::vector<double> um_;   // per component: its uniform window M this race, 0 if not a uniform racer
     std::vector<double> tilted_;
     const IsOpts *is_ = nullptr;
-    std::vector<double> *base_rates_ = nullptr;  // receives the base rates of this race
+    std::vector<double> *hazards_ = nullptr;  // recording: base cumulative hazard of each component
     double log2lr_ = 0.0;
     int qvar_ = -1;
     double qmax_ = 0, qmin_ = 0;
@@ -589,9 +587,18 @@ private:
 
     // t: the time before the step (for the crossing of the bound); full:
     // every component races, the null racers included
+    //
+    // Importance sampling (is_->theta): an exponential component's rate is
+    // r 2^theta; a uniform (clock-bounded) component's hazard 1/(M - d) is
+    // multiplied by a = 2^theta, so its delay has survival (1 - d/M)^a on
+    // the same window [0, M] and is drawn as M (1 - (1 - u)^(1/a)).  Both
+    // keep the log-likelihood of a macro-step linear in theta and 2^theta:
+    // ln q = ln2 theta (wins) - 2^theta (base cumulative hazard) + const.
+    // won: the winner is an exponential or uniform racer (its own draw).
     void race_canonical(uint64_t run, uint32_t step, double t, bool full, double *d_out, int *w_out,
-                        bool *exp_won) {
+                        bool *won_out) {
         const int N = n_.n_components;
+        const float *theta = is_ ? is_->theta : nullptr;
         int zero = -1, wu = -1;
         double du = HUGE_VAL;
         cost(2 * COST_OP * N);  // delay class and leaf of every component
@@ -599,6 +606,7 @@ private:
         for (int c = 0; c < N; ++c) {
             leaf_[c] = 0.0;
             null_[c] = 0;
+            um_[c] = 0.0;
             const int type = n_.locations[loc_[c]].type;
             if (type > SMC_LOC_GOAL) { if (zero < 0) zero = c; continue; }
             double M;
@@ -606,7 +614,10 @@ private:
                 if (M == 0.0) { if (zero < 0) zero = c; continue; }
                 cost(COST_PHILOX + COST_U01 + 3 * COST_OP);
                 smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_UNIFORM, (uint32_t)c);
-                const double d = M * smc_u01(w.v[0], w.v[1]);
+                const double u = smc_u01(w.v[0], w.v[1]);
+                const double th = theta ? (double)theta[loc_[c]] : 0.0;
+                const double d = th == 0.0 ? M * u : M * (1.0 - smc_exp2(smc_log2(1.0 - u) / smc_exp2(th)));
+                um_[c] = M;
                 if (d < du) { du = d; wu = c; }
                 uniform = true;
             } else if (freeze_ && blocked_[c]) {
@@ -627,7 +638,6 @@ private:
                     const double r = rate_of(c);
                     leaf_[c] = r > 0.0 ? r : 0.0;
                 }
-        if (base_rates_) for (int c = 0; c < N; ++c) (*base_rates_)[c] = leaf_[c];
         if (zero >= 0) { *d_out = 0.0; *w_out = zero; return; }
         {
             int P = 1, lv = 0;
@@ -636,7 +646,6 @@ private:
             cost(COST_OP * (P - 1) + 3 * COST_OP * lv + COST_PHILOX + 2 * COST_U01 + COST_LOG2 + COST_DIV +
                  4 * COST_OP);
         }
-        const float *theta = is_ ? is_->theta : nullptr;
         if (theta) {
             // tilted proposal: rate r 2^theta[location] for exponential components
             tilted_.assign(leaf_.begin(), leaf_.end());
@@ -657,8 +666,8 @@ private:
             wexp = tree_.descend(smc_u01(w.v[2], w.v[3]) * Rt);
         }
         if (wu < 0) wu = 0x7FFFFFFF;
-        bool won = false;
-        if (du < dexp || (du == dexp && wu < wexp)) { *d_out = du; *w_out = wu; }
+        bool won = false, uwon = false;
+        if (du < dexp || (du == dexp && wu < wexp)) { *d_out = du; *w_out = wu; uwon = true; }
         else { *d_out = dexp; *w_out = wexp; won = true; }
         // the step that crosses the bound: the left-out null racers' first
         // event after the bound (base rates) may come first; then the step
@@ -682,12 +691,34 @@ private:
                 }
             }
         }
-        if (exp_won) *exp_won = won;
+        if (won_out) *won_out = won || uwon;
         if (theta && *d_out < HUGE_VAL) {
-            // log2 of the likelihood ratio of this race: -theta_w (exponential
-            // winner) + (R' - R) d
-            const double th_w = won ? (double)theta[loc_[*w_out]] : 0.0;
-            log2lr_ = log2lr_ - th_w + (Rt - R) * *d_out;
+            // log2 of the likelihood ratio of this race: -theta_w (the
+            // winner's own draw) + (R' - R) d for the exponential racers
+            // + (1 - a_c) log2(1 - d/M_c) for each tilted uniform racer c
+            // (the uniform terms summed on the race's pairwise tree)
+            const double th_w = (won || uwon) ? (double)theta[loc_[*w_out]] : 0.0;
+            double lr = log2lr_ - th_w + (Rt - R) * *d_out;
+            if (uniform) {
+                dleaf_.assign(N, 0.0);
+                for (int c = 0; c < N; ++c)
+                    if (um_[c] > 0.0 && theta[loc_[c]] != 0.0)
+                        dleaf_[c] = (1.0 - smc_exp2((double)theta[loc_[c]])) * smc_log2(1.0 - *d_out / um_[c]);
+                dead_tree_.build(dleaf_, N);
+                lr = lr + dead_tree_.t[1];
+            }
+            log2lr_ = lr;
+        }
+        if (hazards_ && *d_out < HUGE_VAL) {
+            // recording: each racer's base cumulative hazard over this step
+            // (the step's whole delay, as in its likelihood ratio): r ln2 d
+            // (exponential), -ln(1 - d/M) (uniform)
+            const double LN2 = 0.69314718055994530942;
+            const double dd = *d_out;
+            for (int c = 0; c < N; ++c) {
+                if (leaf_[c] > 0.0) (*hazards_)[c] += LN2 * leaf_[c] * dd;
+                else if (um_[c] > 0.0) (*hazards_)[c] += -LN2 * smc_log2(1.0 - dd / um_[c]);
+            }
         }
     }
 

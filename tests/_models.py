@@ -385,3 +385,12 @@ def random_tree(rng, net: RawNetwork, n_vars: int, depth: int) -> int:
                         random_tree(rng, net, n_vars, depth - 1))
     return net.expr(int(rng.choice(BINARY)), random_tree(rng, net, n_vars, depth - 1),
                     random_tree(rng, net, n_vars, depth - 1))
+
+
+def large_model(name: str) -> str:
+    """A model of the reference's large-model corpus (tests/golden/models/
+    large, gzipped) as XML text."""
+    import gzip
+    from pathlib import Path
+    p = Path(__file__).resolve().parent / "golden" / "models" / "large" / (name + ".gz")
+    return gzip.decompress(p.read_bytes()).decode()

@@ -14,7 +14,6 @@ from cusmc_amd import generators as G
 
 import oracle
 
-REF = Path("/root/reference/xml_files")
 
 # (file, components, variables, locations, edges)
 CORPUS = [
@@ -37,13 +36,22 @@ def test_corpus_models_parse(models_dir, f, nc, nv, nl, ne):
     assert (n.n_components, n.n_vars, n.n_locations, n.n_edges) == (nc, nv, nl, ne)
 
 
-@pytest.mark.skipif(not REF.exists(), reason="reference corpus not mounted")
-@pytest.mark.parametrize("f", ["LargeModels/CSMA_100.xml", "LargeModels/aloha_50.xml",
-                               "LargeModels/fischer_100_29.xml", "agentBaseCovid_350.xml",
-                               "agentBaseCovid_100_1.0.xml", "UppaalBehaviorTest3.xml"])
-def test_large_reference_models_parse(f):
-    net = cs.ParsedNetwork.from_file(REF / f, "")
-    assert net.n_components > 0
+@pytest.mark.parametrize("f,nc,nv,nl,ne", [("CSMA_100.xml", 100, 900, 1800, 5500), ("aloha_50.xml", 50, 3, 250, 400),
+                                            ("fischer_100_29.xml", 100, 202, 400, 500),
+                                            ("agentBaseCovid_350.xml", 350, 2, 2100, 2100),
+                                            ("agentBaseCovid_100_1.0.xml", 100, 2, 600, 600)])
+def test_large_reference_models_parse(f, nc, nv, nl, ne):
+    """The reference's large-model corpus (fixtures: xml_files/LargeModels and
+    the agent-based covid models, gzipped)."""
+    from _models import large_model
+    net = cs.ParsedNetwork.from_string(large_model(f), "")
+    n = net.net
+    assert (n.n_components, n.n_vars, n.n_locations, n.n_edges) == (nc, nv, nl, ne)
+
+
+def test_behavior_test3_parses(models_dir):
+    net = cs.ParsedNetwork.from_file(models_dir / "UppaalBehaviorTest3.xml", "c2.g2")
+    assert net.n_components == 2 and net.goal_count == 1
 
 
 def test_variable_ids_follow_declaration_order(models_dir):

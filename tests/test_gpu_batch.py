@@ -1,10 +1,11 @@
 """Batched multi-instance launches, graph-replayed rounds and the autotuner.
 
-A batch groups instances with uneven component counts by launch shape, runs
-the groups concurrently and reduces every instance on the GPU; each replay of
-its captured graph simulates the next run range of every instance.  Per-run
-results do not depend on the launch, so every per-instance count must equal
-the count of an eager smc_run over the same run range.
+A batch runs instances with uneven component counts in one persistent
+launch (one slots-per-lane value, lanes per run per instance) and reduces
+every instance on the GPU; each replay of its captured graph simulates the
+next run range of every instance.  Per-run results do not depend on the
+launch, so every per-instance count must equal the count of an eager
+smc_run over the same run range.
 """
 from __future__ import annotations
 
@@ -50,7 +51,7 @@ def test_batch_matches_eager_runs(sims):
     finally:
         b.close()
     print(f"[batch] groups={st.groups} runs={st.runs} graph_ms={st.graph_ms:.2f}", flush=True)
-    assert st.runs == n * len(sims) and st.groups >= 3
+    assert st.runs == n * len(sims) and st.groups == 1  # one persistent launch
     for (s, q), r, k in zip(sims, res, range(len(sims))):
         e = s.run(q, seed=100 + k, n_runs=n)
         assert r.hits == e.hits and r.steps == e.steps and r.transitions == e.transitions
@@ -168,3 +169,35 @@ def test_c4_instances_inside_ch_interval_of_reference_algorithm():
         print(f"[c4 ch] instance {k} ({d['kind']} N={d['n']}): gpu {r.estimate:.4f} "
               f"reference-algorithm {ref.estimate:.4f} eps {eps:.4f}", flush=True)
         assert abs(r.estimate - ref.estimate) <= eps
+
+
+def test_autotune_choice_persists(tmp_path, monkeypatch):
+    """The tuned shape is written to the cache file (SMC_TUNE_CACHE) and a
+    second model of the same network and query takes it from there without
+    timing anything; the waves-per-CU axis is part of the search."""
+    import time
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("SMC_TUNE_CACHE", str(cache))
+    xml, _ = G.fischer_network(16)
+    q = cs.reach_query(60.0)
+    a = cs.GpuSimulator(cs.ParsedNetwork.from_string(xml, "P0.cs"))
+    b = cs.GpuSimulator(cs.ParsedNetwork.from_string(xml, "P0.cs"))
+    try:
+        t0 = time.perf_counter()
+        first = a.autotune(q, seed=1, sample_runs=8192)
+        t_first = time.perf_counter() - t0
+        lines = cache.read_text().split("\n")
+        assert len([l for l in lines if l]) == 1
+        t0 = time.perf_counter()
+        second = b.autotune(q, seed=1, sample_runs=8192)
+        t_second = time.perf_counter() - t0
+        assert (second.lanes_per_run, second.waves_per_cu, second.occupancy, second.tables) == \
+               (first.lanes_per_run, first.waves_per_cu, first.occupancy, first.tables)
+        assert t_second < t_first / 5, (t_first, t_second)
+        assert len([l for l in cache.read_text().split("\n") if l]) == 1
+        r = b.run(q, seed=3, n_runs=4000, per_run=("digest",))
+        e = a.run(q, seed=3, n_runs=4000, lanes_per_run=16, per_run=("digest",))
+        assert np.array_equal(r.per_run["digest"], e.per_run["digest"])
+    finally:
+        a.close()
+        b.close()

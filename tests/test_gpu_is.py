@@ -339,3 +339,40 @@ def test_trained_proposal_gains_over_untrained():
     finally:
         s.close()
         sim.close()
+
+
+def test_trained_window_and_rate_tilts_gain_on_race8():
+    """race-8 mixes uniform windows (L0, L2: invariant x <= U, guard x >= L)
+    and exponential locations (L1, L3).  Pr[<=20](<> count > 54) is about
+    4e-5: the proposal trained with window and rate tilts against plain
+    simulation of 4e7 runs."""
+    net = cs.ParsedNetwork.from_string(G.race_network(8)[0], "")
+    q = cs.var_query(net.variable_id("count"), 20.0, threshold=54)
+    sim = cs.GpuSimulator(net)
+    s = cs.ImportanceSampler(sim, q, rounds=5, runs_per_round=20000, epochs=20, seed=7)
+    try:
+        plain = sim.run(q, seed=5, n_runs=40_000_000)
+        sp = math.sqrt(plain.estimate * (1 - plain.estimate) / plain.n_runs)
+        n = 32768
+        before = s.sample(seed=31, n_runs=n, use_mlp=True, lockstep=True, early_stop=True)
+        rounds = s.train()
+        for r in rounds:
+            print(f"[is race8] round level={r['level']} elites={r['elites']} hits={r['hits']} "
+                  f"est={r['estimate']:.3e} ms={r['ms']:.0f}", flush=True)
+        after = s.estimate(seed=31, batch_runs=n, max_runs=n, rel_target=0.0001)
+        se = after.half_width / 1.959963984540054
+        print(f"[is race8] plain {plain.hits}/{plain.n_runs} = {plain.estimate:.3e}; untrained hits "
+              f"{before.hits}/{n}; trained hits {after.hits}/{after.n_runs} est {after.estimate:.3e} "
+              f"rel_hw {after.rel_half_width:.3f}", flush=True)
+        assert plain.hits >= 400
+        assert before.hits <= 10
+        assert after.hits >= 50 * max(before.hits, 1)
+        assert abs(after.estimate - plain.estimate) < 4.0 * math.sqrt(se ** 2 + sp ** 2)
+        # relative variance per run: the trained proposal's is several times
+        # smaller than plain simulation's (1 - p) / p
+        p = plain.estimate
+        rel_var_is = (after.rel_half_width / 1.959963984540054) ** 2 * after.n_runs
+        assert rel_var_is < (1 - p) / p / 5, (rel_var_is, (1 - p) / p)
+    finally:
+        s.close()
+        sim.close()
