@@ -60,7 +60,7 @@ Plan plan_launch(int P, int max_vars, int lanes, int occ, int tables, uint32_t t
     pl.C = P / S;
     if (pl.C > 16) throw std::runtime_error("limit: more than 16 components per lane; use more lanes per run");
     pl.R = 64 / S;
-    pl.stride = (max_vars > 0 ? max_vars : 1) | 1;
+    pl.stride = (max_vars + 1) | 1;  // clock time, then the variables
     // per wave: variables of the R packed runs, then two doubles per lane
     // for each butterfly level of the race tree
     int levels = 0;
@@ -230,8 +230,12 @@ int smc_model_create(const smc_network *net, int device, smc_model **out) {
         hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto &e : m->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
 
-        // one arena for every table
+        // one arena for every table; variables start in their stored form
+        // (a clock x = v0 as the reset stamp -v0)
         const smc::ModelImage &im = m->img;
+        std::vector<double> var_start(im.var_init);
+        for (size_t v = 0; v < var_start.size() && v < im.var_clock.size(); ++v)
+            if (im.var_clock[v]) var_start[v] = -var_start[v];
         struct Part { const void *src; size_t bytes; size_t off; };
         std::vector<Part> parts = {
             {im.locs.data(), im.locs.size() * sizeof(DevLoc), 0},
@@ -241,9 +245,9 @@ int smc_model_create(const smc_network *net, int device, smc_model **out) {
             {im.upds.data(), im.upds.size() * sizeof(DevUpd), 0},
             {im.consts.data(), im.consts.size() * sizeof(double), 0},
             {im.code.data(), im.code.size() * sizeof(DevInsn), 0},
-            {im.var_init.data(), im.var_init.size() * sizeof(double), 0},
+            {var_start.data(), var_start.size() * sizeof(double), 0},
             {im.comp_init_loc.data(), im.comp_init_loc.size() * sizeof(int32_t), 0},
-            {im.clocks.data(), im.clocks.size() * sizeof(uint16_t), 0},
+            {im.var_clock.data(), im.var_clock.size() * sizeof(uint8_t), 0},
             {im.comp_first_loc.data(), im.comp_first_loc.size() * sizeof(int32_t), 0},
         };
         size_t total = 0;
@@ -264,7 +268,7 @@ int smc_model_create(const smc_network *net, int device, smc_model **out) {
         d.code = (decltype(d.code))at(6);
         d.var_init = (decltype(d.var_init))at(7);
         d.comp_init_loc = (decltype(d.comp_init_loc))at(8);
-        d.clocks = (decltype(d.clocks))at(9);
+        d.var_clock = (decltype(d.var_clock))at(9);
         d.comp_first_loc = (decltype(d.comp_first_loc))at(10);
         d.n_comp = im.n_comp;
         d.n_vars = im.n_vars;
@@ -696,7 +700,7 @@ int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_bat
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (hipEvent_t *e : {&b->ev0, &b->ev1}) hip_check(hipEventCreate(e), "hipEventCreate");
         const int cu = items[0].model->cu_count;
-        b->stride = max_vars | 1;
+        b->stride = (max_vars + 1) | 1;  // clock time, then the variables
         b->bufs.resize(n_items);
         uint64_t maxr = 0;
         for (int i = 0; i < n_items; ++i) {

@@ -41,7 +41,8 @@ public:
         const smc_expr_node &x = n_.exprs[e];
         if (x.op == SMC_E_LIT) return SMC_REF_MAKE(SMC_REF_CONST, const_index(x.value));
         if (x.op == SMC_E_RANDOM) return SMC_REF_MAKE(SMC_REF_CONST, const_index(1.0));
-        if (x.op == SMC_E_VAR) return SMC_REF_MAKE(SMC_REF_VAR, (uint32_t)x.var);
+        if (x.op == SMC_E_VAR)
+            return SMC_REF_MAKE(n_.var_kind[x.var] == SMC_VAR_CLOCK ? SMC_REF_CLOCK : SMC_REF_VAR, (uint32_t)x.var);
         std::vector<DevInsn> prog;
         int maxreg = 0;
         uint16_t r = emit(e, 0, prog, &maxreg);
@@ -105,7 +106,7 @@ private:
         if (x.op == SMC_E_RANDOM) return SMC_OPND(SMC_OPND_CONST, const_index(1.0));
         if (x.op == SMC_E_VAR) {
             if (x.var > SMC_MAX_OPND_INDEX) throw std::runtime_error("limit: variable index above 16383 in an expression");
-            return SMC_OPND(SMC_OPND_VAR, (uint32_t)x.var);
+            return SMC_OPND(n_.var_kind[x.var] == SMC_VAR_CLOCK ? SMC_OPND_CLOCK : SMC_OPND_VAR, (uint32_t)x.var);
         }
         int ar = smc_op_arity(x.op);
         const int kids[3] = {x.a, x.b, x.c};
@@ -148,11 +149,7 @@ ModelImage compile_model(const smc_network &n) {
 
     img.var_init.assign(n.var_init, n.var_init + n.n_vars);
     img.var_kind.assign(n.var_kind, n.var_kind + n.n_vars);
-    for (int v = 0; v < n.n_vars; ++v)
-        if (n.var_kind[v] == SMC_VAR_CLOCK) {
-            if (v > 65535) throw std::runtime_error("limit: clock variable index above 65535");
-            img.clocks.push_back((uint16_t)v);
-        }
+    for (int v = 0; v < n.n_vars; ++v) img.var_clock.push_back(n.var_kind[v] == SMC_VAR_CLOCK ? 1 : 0);
     img.comp_init_loc.assign(n.comp_init_loc, n.comp_init_loc + n.n_components);
     img.comp_first_loc.assign(n.comp_first_loc, n.comp_first_loc + n.n_components + 1);
 
@@ -271,7 +268,8 @@ ModelImage compile_model(const smc_network &n) {
             DevGuard g{};
             g.var = (uint16_t)C.var;
             g.op = (uint8_t)C.op;
-            g.clock_free = n.var_kind[C.var] != SMC_VAR_CLOCK && !reads_clock(C.rhs);
+            g.flags = (uint8_t)((n.var_kind[C.var] != SMC_VAR_CLOCK && !reads_clock(C.rhs) ? SMC_GF_CLOCK_FREE : 0) |
+                                (n.var_kind[C.var] == SMC_VAR_CLOCK ? SMC_GF_CLOCK : 0));
             g.rhs = lw.ref(C.rhs, 0.0);
             img.guards.push_back(g);
         }
@@ -286,6 +284,7 @@ ModelImage compile_model(const smc_network &n) {
             if (U.var > 65535) throw std::runtime_error("limit: assigned variable index above 65535");
             DevUpd u{};
             u.var = (uint16_t)U.var;
+            u.clock = n.var_kind[U.var] == SMC_VAR_CLOCK ? 1 : 0;
             u.expr = lw.ref(U.expr, 0.0);
             double cv = 0.0;
             const int f = E.channel < 0 ? form(U, &cv) : F_OTHER;
@@ -362,19 +361,20 @@ ModelImage compile_model(const smc_network &n) {
     if (img.consts.empty()) img.consts.push_back(0.0);
     if (img.code.empty()) img.code.push_back(DevInsn{});
     if (img.var_init.empty()) img.var_init.push_back(0.0);
-    if (img.clocks.empty()) img.clocks.push_back(0);
+    if (img.var_clock.empty()) img.var_clock.push_back(0);
     return img;
 }
 
 double eval_ref_host(const ModelImage &m, uint32_t ref, const double *vars) {
+    // vars: values (clocks by value, not as stamps)
     uint32_t kind = SMC_REF_KIND(ref);
     if (kind == SMC_REF_CONST) return m.consts[ref & 0x3FFFFFFFu];
-    if (kind == SMC_REF_VAR) return vars[ref & 0x3FFFFFFFu];
+    if (kind == SMC_REF_VAR || kind == SMC_REF_CLOCK) return vars[ref & 0x3FFFFFFFu];
     uint32_t first = ref & 0xFFFFFFu, count = (ref >> 24) & 0x3Fu;
     double reg[SMC_NREGS] = {0, 0, 0, 0};
     auto fetch = [&](uint16_t o) -> double {
         uint32_t k = o >> 14, i = o & 0x3FFFu;
-        return k == SMC_OPND_REG ? reg[i & 3] : k == SMC_OPND_VAR ? vars[i] : m.consts[i];
+        return k == SMC_OPND_REG ? reg[i & 3] : (k == SMC_OPND_VAR || k == SMC_OPND_CLOCK) ? vars[i] : m.consts[i];
     };
     for (uint32_t k = 0; k < count; ++k) {
         const DevInsn &in = m.code[first + k];

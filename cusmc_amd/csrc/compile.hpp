@@ -19,7 +19,7 @@ struct ModelImage {
     std::vector<double> var_init;
     std::vector<int32_t> comp_init_loc;
     std::vector<int32_t> comp_first_loc;
-    std::vector<uint16_t> clocks;
+    std::vector<uint8_t> var_clock;     // per variable: 1 for a clock
     std::vector<int32_t> var_kind;
     int n_comp = 0, n_vars = 0, n_leaves = 1;
     bool freeze = true;   // no location's rate reads a clock

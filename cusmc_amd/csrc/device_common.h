@@ -178,7 +178,7 @@ __device__ __forceinline__ DevModelT<SmcLdsTables> lds_model(const DevModel &mg,
     m.code = in_lds(mg.code, base, tab);
     m.var_init = mg.var_init;
     m.comp_init_loc = mg.comp_init_loc;
-    m.clocks = mg.clocks;
+    m.var_clock = mg.var_clock;
     m.comp_first_loc = mg.comp_first_loc;
     m.n_comp = mg.n_comp;
     m.n_vars = mg.n_vars;
@@ -208,7 +208,7 @@ __device__ __forceinline__ double eval_code(uint32_t ref, const double *vars, PC
             const uint16_t x = s == 0 ? in.a : s == 1 ? in.b : in.c;
             const uint32_t kind = x >> 14, i = x & 0x3FFFu;
             o[s] = kind == SMC_OPND_REG ? (i == 0 ? r0 : i == 1 ? r1 : i == 2 ? r2 : r3)
-                 : kind == SMC_OPND_VAR ? vars[i] : consts[i];
+                 : kind == SMC_OPND_VAR ? vars[i] : kind == SMC_OPND_CLOCK ? vars[-1] - vars[i] : consts[i];
         }
         const double v = smc_apply_op(in.op, o[0], o[1], o[2]);
         r0 = in.dst == 0 ? v : r0;
@@ -225,6 +225,7 @@ __device__ __forceinline__ double eval_ref(uint32_t ref, const double *vars, con
     const uint32_t kind = ref >> 30;
     if (kind == SMC_REF_CONST) return m.consts[ref & 0x3FFFFFFFu];
     if (kind == SMC_REF_VAR) return vars[ref & 0x3FFFFFFFu];
+    if (kind == SMC_REF_CLOCK) return vars[-1] - vars[ref & 0x3FFFFFFFu];
     return eval_code(ref, vars, m.consts, m.code);
 }
 
@@ -277,7 +278,8 @@ __device__ __forceinline__ uint64_t enabled_mask(int loc, int recv_chan, const d
         bool ok = true;
         for (int g = 0; g < E.guard_n && ok; ++g) {
             const DevGuard G = load_rec(m.guards + E.guard_first + g);
-            ok = guard_holds(G.op, vars[G.var], eval_ref(G.rhs, vars, m));
+            const double x = (G.flags & SMC_GF_CLOCK) ? vars[-1] - vars[G.var] : vars[G.var];
+            ok = guard_holds(G.op, x, eval_ref(G.rhs, vars, m));
         }
         if (ok) mask |= 1ull << i;
     }
@@ -355,7 +357,7 @@ __device__ __forceinline__ int fire_edge(int e, double *vars, const M &m, LaneQu
             if (v > q.qmax) q.qmax = v;
             if (v < q.qmin) q.qmin = v;
         }
-        vars[U.var] = v;
+        vars[U.var] = U.clock ? vars[-1] - v : v;
     }
     if (E.dst_goal) goal = true;
     if (E.channel > 0) chan = E.channel;
@@ -387,7 +389,8 @@ __device__ __forceinline__ uint32_t blocked_slots(const int (&loc)[C], const dou
                 bool e_blk = false;
                 for (int g = 0; g < E.guard_n && !e_blk; ++g) {
                     const DevGuard G = load_rec(m.guards + E.guard_first + g);
-                    if (G.clock_free) e_blk = !guard_holds(G.op, vars[G.var], eval_ref(G.rhs, vars, m));
+                    if (G.flags & SMC_GF_CLOCK_FREE)
+                        e_blk = !guard_holds(G.op, vars[G.var], eval_ref(G.rhs, vars, m));
                 }
                 blk = e_blk;
             }
@@ -504,7 +507,7 @@ __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int
             for (int k = 0; k < E.upd_n; ++k) {
                 const DevUpd U = load_rec(m.upds + E.upd_first + k);
                 if (U.kind == SMC_U_ADD) atomicAdd(&vars[U.var], U.value);
-                else vars[U.var] = U.value;
+                else vars[U.var] = U.clock ? vars[-1] - U.value : U.value;
             }
             if (E.dst_goal) goal = true;
             loc[j] = E.dst;

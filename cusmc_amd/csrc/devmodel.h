@@ -9,13 +9,21 @@
 //
 //   expression reference (uint32):  bits 31..30 kind
 //       SMC_REF_CONST  index into consts[]
-//       SMC_REF_VAR    variable id
+//       SMC_REF_VAR    variable id (an integer variable)
 //       SMC_REF_CODE   bits 0..23 first instruction, bits 24..29 count
+//       SMC_REF_CLOCK  variable id of a clock
 //
 //   instruction (8 bytes): op, dst register, up to three operands; an
-//   operand (uint16) is a register r0..r3, a variable or a constant:
-//       bits 15..14 kind (0 reg, 1 var, 2 const), bits 13..0 index.
-//   The value of an expression is register r0 after its last instruction.
+//   operand (uint16) is a register r0..r3, a variable, a constant or a
+//   clock: bits 15..14 kind (0 reg, 1 var, 2 const, 3 clock), bits 13..0
+//   index.  The value of an expression is register r0 after its last
+//   instruction.
+//
+// Clocks are stored as reset stamps: a run's variable array holds, for a
+// clock x, the time s at which x would have been 0, and its value is t - s
+// with t the run's clock time, kept just before the variables (vars[-1]).
+// Time then advances with one store, however many clocks the network has;
+// assigning x := v stores s = t - v.
 #pragma once
 
 #include <stdint.h>
@@ -23,6 +31,7 @@
 #define SMC_REF_CONST 0u
 #define SMC_REF_VAR 1u
 #define SMC_REF_CODE 2u
+#define SMC_REF_CLOCK 3u
 #define SMC_REF_KIND(r) ((r) >> 30)
 #define SMC_REF_MAKE(kind, payload) (((uint32_t)(kind) << 30) | (uint32_t)(payload))
 #define SMC_REF_CODE_MAKE(first, count) (((uint32_t)SMC_REF_CODE << 30) | ((uint32_t)(count) << 24) | (uint32_t)(first))
@@ -30,6 +39,7 @@
 #define SMC_OPND_REG 0u
 #define SMC_OPND_VAR 1u
 #define SMC_OPND_CONST 2u
+#define SMC_OPND_CLOCK 3u
 #define SMC_OPND(kind, idx) ((uint16_t)(((kind) << 14) | (idx)))
 
 #define SMC_NREGS 4
@@ -96,10 +106,16 @@ struct DevEdge {            // 32 bytes
     uint32_t pad1;
 };
 
+// DevGuard.flags
+enum {
+    SMC_GF_CLOCK_FREE = 1,  // neither var nor rhs reads a clock: fixed between transitions
+    SMC_GF_CLOCK = 2        // var is a clock
+};
+
 struct DevGuard {           // variables[var] op rhs
     uint16_t var;
     uint8_t op;             // smc_cons_op, never SMC_C_NE
-    uint8_t clock_free;     // neither var nor rhs reads a clock: fixed between transitions
+    uint8_t flags;          // SMC_GF_*
     uint32_t rhs;
 };
 
@@ -112,7 +128,7 @@ enum { SMC_U_EXPR = 0, SMC_U_ADD = 1, SMC_U_SET = 2 };
 struct DevUpd {             // 16 bytes
     uint16_t var;
     uint8_t kind;           // SMC_U_*
-    uint8_t pad;
+    uint8_t clock;          // var is a clock (stored as a reset stamp)
     uint32_t expr;
     double value;           // the added constant (ADD) or the value (SET)
 };
@@ -148,9 +164,9 @@ struct DevModelT {
     typename A::template ptr<DevUpd> upds;
     typename A::template ptr<double> consts;
     typename A::template ptr<DevInsn> code;
-    const SMC_GLOBAL double *var_init;
+    const SMC_GLOBAL double *var_init;     // stored form at time 0: initial value, -initial value for a clock
     const SMC_GLOBAL int32_t *comp_init_loc;
-    const SMC_GLOBAL uint16_t *clocks;     // ids of clock variables
+    const SMC_GLOBAL uint8_t *var_clock;   // [n_vars] 1: the variable is a clock
     const SMC_GLOBAL int32_t *comp_first_loc;  // [n_comp + 1] location range of each component
     int32_t n_comp;
     int32_t n_vars;

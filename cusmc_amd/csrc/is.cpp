@@ -568,7 +568,7 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         s->C = P / s->S;
         if (s->C > 8) throw std::runtime_error("internal: more than 8 slots per lane");
         s->R = 64 / s->S;
-        s->stride = (img.n_vars > 0 ? img.n_vars : 1) | 1;
+        s->stride = (img.n_vars + 1) | 1;  // clock time, then the variables
 
         s->n_locs = (int)img.locs.size();
         s->n_vars = img.n_vars;

@@ -64,7 +64,10 @@ __device__ __forceinline__ void write_features(const IsParams &P, uint16_t *f, c
         }
     }
     const int base = P.m.n_locs;
-    for (int v = sl; v < P.m.n_vars; v += S) f[base + v] = f32_to_bf16((float)(vars[v] * P.var_scale[v]));
+    for (int v = sl; v < P.m.n_vars; v += S) {
+        const double x = P.m.var_clock[v] ? vars[-1] - vars[v] : vars[v];
+        f[base + v] = f32_to_bf16((float)(x * P.var_scale[v]));
+    }
     if (sl == 0) {
         f[base + P.m.n_vars] = f32_to_bf16((float)(t / P.bound));
         for (int k = base + P.m.n_vars + 1; k < P.fpad; ++k) f[k] = 0;
@@ -107,7 +110,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
     const uint64_t seg_mask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << seg_base);
     const int row = group * (64 / S) + lane / S;
     if (row >= P.n_rows) return;  // whole segments only
-    double *vars = wlds + (size_t)(lane / S) * P.var_stride;
+    double *vars = wlds + (size_t)(lane / S) * P.var_stride + 1;  // [clock time][variables]
     const IsRows &st = P.st;
     const int n = m.n_comp;
     const double INF = __builtin_huge_val();
@@ -126,6 +129,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
     LaneQuery q{P.qvar, st.qmax[row], st.qmin[row]};
     uint64_t digest = st.digest[row];
     for (int v = sl; v < m.n_vars; v += S) vars[v] = st.vars[(size_t)row * m.n_vars + v];
+    if (sl == 0) vars[-1] = t;
     int loc[C];
     // recording build: per slot the starting location, the base cumulative
     // hazard over the macro-step's races and the wins, in LDS ([slot][lane])
@@ -242,7 +246,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                             for (int i = 0; i < L.n_inv; ++i) {
                                 const DevInv iv = load_rec(m.invs + L.inv_first + i);
                                 const double bound = iv.bound_const ? iv.bound_value : eval_ref(iv.bound, vars, m);
-                                double ttb = bound - vars[iv.var];
+                                double ttb = bound - (vars[-1] - vars[iv.var]);
                                 if (iv.strict) ttb -= 1e-6;
                                 if (ttb >= 0.0) { M = ttb < M ? ttb : M; bounded = true; }
                             }
@@ -263,10 +267,9 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                         }
                     }
                 }
-                // frozen components race only beside a uniform racer or in
-                // a full race (as in the run kernel)
-                const bool uniform = (__ballot(bmask != 0) & seg_mask) != 0;
-                if (uniform || full) {
+                // frozen components race only in a full race (as in the run
+                // kernel)
+                if (full) {
                     for (uint32_t b = deadmask; b; b &= b - 1) {
                         const int j = __ffs(b) - 1;
                         const int lj = slot_get<C>(loc, j);
@@ -361,13 +364,12 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     else { d = dexp; w = wexp; exp_won = true; }
                     // the step that crosses the bound: the left-out null
                     // racers' first event after it (base rates, no tilt)
-                    if (!uniform && !full && d < INF && !(t + d < P.bound) &&
-                        (__ballot(deadmask != 0) & seg_mask)) {
+                    if (!full && d < INF && !(t + d < P.bound) && (__ballot(deadmask != 0) & seg_mask)) {
                         const double Rd = dead_total<C>(deadmask, loc, vars, m, S);
                         if (Rd > 0.0) {
                             const smc_u32x4 rw = smc_draw(P.seed, run, step, SMC_P_DEAD, 0);
                             const double td = P.bound + (-smc_log2(smc_u01(rw.v[0], rw.v[1])) / Rd);
-                            if (td < t + d) { d = td - t; w = -1; exp_won = false; }
+                            if (td < t + d) { d = td - t; w = -1; exp_won = false; u_won = false; }
                         }
                     }
                     if (theta && d < INF) {
@@ -416,7 +418,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     }
                 }
                 if (!(d < INF)) { t = INF; end = true; break; }
-                for (int i = sl; i < m.n_clocks; i += S) vars[m.clocks[i]] += d;
+                if (sl == 0) vars[-1] = t + d;  // clocks advance
                 wave_sync();
                 if (w < 0) break;  // a null event of the left-out racers
                 // the winner's transition, once, on the lane holding it
@@ -482,7 +484,8 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
         if (!hit && !P.reach) hit = P.is_max ? (qmax > P.threshold) : (qmin < P.threshold);
         if (P.trace) {
             digest = smc_mix(digest, smc_digest_bits(t));
-            for (int v = 0; v < m.n_vars; ++v) digest = smc_mix(digest, smc_digest_bits(vars[v]));
+            for (int v = 0; v < m.n_vars; ++v)
+                digest = smc_mix(digest, smc_digest_bits(m.var_clock[v] ? vars[-1] - vars[v] : vars[v]));
             for (int c = 0; c < n; ++c) {
                 const int lv = __shfl(slot_get<C>(loc, c % C), seg_base + c / C);
                 digest = smc_mix(digest, (uint64_t)(uint32_t)lv);

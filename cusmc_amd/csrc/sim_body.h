@@ -15,8 +15,10 @@
 //     (blocked: lane l holds components l*C .. l*C+C-1).  A workgroup is one
 //     wave, so no workgroup barrier is ever needed; lanes of a segment talk
 //     through cross-lane shuffles, ballots and the segment's LDS block.
-//   * Variables and clocks of each run live in LDS as doubles; clocks and
-//     global time accumulate in FP64.
+//   * Variables and clocks of each run live in LDS as doubles, clocks as
+//     reset stamps beside the run's clock time (a clock's value is the time
+//     minus its stamp), so advancing time is one store; time accumulates in
+//     FP64.
 //   * The race: cuSMC re-samples every component's delay every step
 //     (simulation.cu:513-540) and takes the minimum.  Here the same law is
 //     sampled with far fewer draws: zero-delay components (branchpoint,
@@ -58,8 +60,8 @@ namespace {
 // them there: the race and the transitions are chains of dependent table
 // reads, and LDS answers them several times sooner than the caches.
 //
-// LDS: [tables: lp.table_bytes, TAB only] then per wave [variables of the
-// wave's 64/S runs: var_stride doubles each][race-tree levels: 2 log2(S)
+// LDS: [tables: lp.table_bytes, TAB only] then per wave [clock time and
+// variables of the wave's 64/S runs: var_stride doubles each][race-tree levels: 2 log2(S)
 // doubles per lane, level-major]; lp.wave_lds covers the largest of these
 // over the launch's instances.
 template <int C, bool TRACE, bool TAB>
@@ -96,7 +98,8 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
     const int sl = lane & (S - 1);
     const int seg_base = lane - sl;
     const uint64_t seg_mask = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << seg_base);
-    double *vars = wbase + (size_t)(lane / S) * lp.var_stride;
+    // a run's block: [clock time][variables] (clocks as reset stamps)
+    double *vars = wbase + (size_t)(lane / S) * lp.var_stride + 1;
     // per lane: the race tree's partial sums at each butterfly level (own at
     // [64 (2k)], sibling block's at [64 (2k + 1)]), kept in LDS rather than
     // registers from the sum to the descent; level-major, so the 64 lanes
@@ -141,6 +144,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                 q.qmax = p.qinit; q.qmin = p.qinit;
                 digest = TRACE ? smc_mix(0, run) : 0;
                 for (int v = sl; v < m.n_vars; v += S) vars[v] = m.var_init[v];
+                if (sl == 0) vars[-1] = 0.0;
 #pragma unroll
                 for (int j = 0; j < C; ++j) {
                     const int c = sl * C + j;
@@ -200,7 +204,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                                 for (int i = 0; i < L.n_inv; ++i) {
                                     const DevInv iv = load_rec(m.invs + L.inv_first + i);
                                     const double bound = iv.bound_const ? iv.bound_value : eval_ref(iv.bound, vars, m);
-                                    double ttb = bound - vars[iv.var];
+                                    double ttb = bound - (vars[-1] - vars[iv.var]);
                                     if (iv.strict) ttb -= 1e-6;
                                     if (ttb >= 0.0) { M = ttb < M ? ttb : M; bounded = true; }
                                 }
@@ -217,14 +221,14 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                         }
                     }
                     // A frozen component (blocked, exponential) only makes
-                    // null steps until the next transition.  With every racer
-                    // exponential, leaving it out of the race keeps the law of
-                    // everything else (memoryless races, rates constant between
-                    // transitions); a uniform racer is redrawn at every step,
-                    // so then it races as in the reference, as it does when
-                    // the step decides the outcome (full).
-                    const bool uniform = (__ballot(bmask != 0) & seg_mask) != 0;
-                    if (uniform || full) {
+                    // null steps until the next transition.  Leaving it out of
+                    // the race keeps the law of everything else: exponential
+                    // racers are memoryless (rates constant between
+                    // transitions), and a uniform racer redrawn at every step
+                    // has hazard 1/(W - tau), a function of the absolute time
+                    // alone (W: the end of its window).  It races as in the
+                    // reference when the step decides the outcome (full).
+                    if (full) {
                         for (uint32_t b = deadmask; b; b &= b - 1) {
                             const int j = __ffs(b) - 1;
                             const DevLoc L = load_rec(m.locs + slot_get<C>(loc, j));
@@ -317,8 +321,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                         else { d = dexp; w = wexp; }
                         // the step that crosses the bound: the left-out null
                         // racers' first event after the bound may come first
-                        if (!uniform && !full && d < INF && !(t + d < p.bound) &&
-                            (__ballot(deadmask != 0) & seg_mask)) {
+                        if (!full && d < INF && !(t + d < p.bound) && (__ballot(deadmask != 0) & seg_mask)) {
                             const double Rd = dead_total<C>(deadmask, loc, vars, m, S);
                             if (Rd > 0.0) {
                                 const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_DEAD, 0);
@@ -328,8 +331,9 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                         }
                     }
                     if (!(d < INF)) { t = INF; end = true; break; }
-                    // clocks advance (find_minimum_delay :632-651)
-                    for (int i = sl; i < m.n_clocks; i += S) vars[m.clocks[i]] += d;
+                    // clocks advance (find_minimum_delay :632-651): one store of
+                    // the run's clock time
+                    if (sl == 0) vars[-1] = t + d;
                     wave_sync();
                     if (w < 0) break;  // a null event of the left-out racers
                     // the winner's transition (check_enabled_edges :268-305,
@@ -396,7 +400,8 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
             const double value = p.is_max ? qmax : qmin;
             if (TRACE) {
                 digest = smc_mix(digest, smc_digest_bits(t));
-                for (int v = 0; v < m.n_vars; ++v) digest = smc_mix(digest, smc_digest_bits(vars[v]));
+                for (int v = 0; v < m.n_vars; ++v)
+                    digest = smc_mix(digest, smc_digest_bits(m.var_clock[v] ? vars[-1] - vars[v] : vars[v]));
                 for (int c = 0; c < n; ++c) {
                     const int lv = __shfl(slot_get<C>(loc, c % C), seg_base + c / C);
                     digest = smc_mix(digest, (uint64_t)(uint32_t)lv);

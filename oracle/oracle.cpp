@@ -16,6 +16,9 @@
 //       otherwise exponential, delay -log2(u)/rate      simulation.cu:522-540
 //     minimum delay, lowest component index on ties     simulation.cu:593-625
 //     clocks advance by it                              simulation.cu:632-651
+//       (stored as reset stamps: a clock's value is the clock time minus
+//       its stamp, the representation of the kernels; assigning x := v
+//       stores stamp = time - v)
 //     the winner fires one enabled non-receive edge     simulation.cu:654-667
 //       weighted by integer-truncated weights           simulation.cu:9-79
 //       updates in order, tracking the query variable   simulation.cu:101-134
@@ -40,8 +43,8 @@
 //     descending the tree with u2*total; bounded components draw uniforms.
 //     Components that can only make null steps until the next transition
 //     (frozen: exponential, every spontaneous edge blocked by a clock-free
-//     conjunct) leave an all-exponential race; the step that crosses the
-//     bound draws their first event after the bound.
+//     conjunct) leave the race; the step that crosses the bound draws their
+//     first event after the bound.
 //     Same Philox stream, same operations, same bits as the GPU.
 // Resolutions of undefined behaviour (by intent): all variables and all
 // clocks are tracked (cuSMC: 3), a receiver picks with its own stream,
@@ -196,6 +199,7 @@ public:
         um_.resize(n.n_components);
         loc_.resize(n.n_components);
         vars_.resize(n.n_vars > 0 ? n.n_vars : 1);
+        stamp_.assign(n.n_vars > 0 ? n.n_vars : 1, 0.0);
         recv_.resize(n.n_components);
         for (int v = 0; v < n.n_vars; ++v)
             if (n.var_kind[v] == SMC_VAR_CLOCK) clocks_.push_back(v);
@@ -262,7 +266,16 @@ public:
         std::vector<double> hz;
         std::vector<uint32_t> wins;
         const int N = n_.n_components;
-        for (int v = 0; v < n_.n_vars; ++v) vars_[v] = n_.var_init[v];
+        // clocks are kept as reset stamps (value = clock time - stamp), the
+        // representation the kernels use; vars_ holds every value
+        tclk_ = 0.0;
+        for (int v = 0; v < n_.n_vars; ++v) {
+            vars_[v] = n_.var_init[v];
+            if (n_.var_kind[v] == SMC_VAR_CLOCK) {
+                stamp_[v] = -n_.var_init[v];
+                vars_[v] = tclk_ - stamp_[v];
+            }
+        }
         for (int c = 0; c < N; ++c) loc_[c] = n_.comp_init_loc[c];
         const bool reach = q_.kind == SMC_QUERY_REACH;
         const int qvar = reach ? -1 : q_.var;
@@ -369,8 +382,11 @@ public:
             if (record && won) ++wins[w];
             if (!(d < HUGE_VAL)) { t = HUGE_VAL; break; }
 
-            for (int v : clocks_) vars_[v] += d;
-            cost(COST_OP * (double)clocks_.size() + 2 * COST_OP);  // clocks, time
+            // clocks advance: the clock time moves, every clock's value is
+            // the clock time minus its stamp
+            tclk_ = t + d;
+            for (int v : clocks_) vars_[v] = tclk_ - stamp_[v];
+            cost(2 * COST_OP);  // clock time, time
             std::vector<int> en;
             if (w >= 0) enabled(loc_[w], 0, &en);
             if (!en.empty()) {
@@ -409,6 +425,8 @@ private:
     int mode_;
     uint32_t max_steps_;
     std::vector<double> leaf_, vars_;
+    std::vector<double> stamp_;  // clocks: reset stamps
+    double tclk_ = 0.0;          // clock time: the time of the last advance
     std::vector<int> loc_, clocks_;
     std::vector<std::vector<int>> recv_;
     RaceTree tree_, base_tree_, dead_tree_;
@@ -554,7 +572,12 @@ private:
                 if (v > qmax_) qmax_ = v;
                 if (v < qmin_) qmin_ = v;
             }
-            vars_[U.var] = v;
+            if (n_.var_kind[U.var] == SMC_VAR_CLOCK) {
+                stamp_[U.var] = tclk_ - v;
+                vars_[U.var] = tclk_ - stamp_[U.var];
+            } else {
+                vars_[U.var] = v;
+            }
         }
         loc_[c] = E.dst;
         if (n_.locations[E.dst].type == SMC_LOC_GOAL) *goal = true;
@@ -627,12 +650,14 @@ private:
                 leaf_[c] = r > 0.0 ? r : 0.0;
             }
         }
-        // A component that cannot fire only makes null steps: with every
-        // racer exponential it leaves the race without changing the law of
-        // the rest (memoryless races); beside a uniform racer, which is
-        // redrawn at every step, it races as in cuSMC, and so it does when
-        // the step itself decides the outcome (full).
-        if (uniform || full)
+        // A component that cannot fire only makes null steps.  Leaving it
+        // out keeps the law of every other event: exponential racers are
+        // memoryless, and a uniform racer redrawn on (0, W - tau] at every
+        // step has hazard 1/(W - tau), a function of the absolute time tau
+        // alone (W: the absolute end of its window), so its firing time
+        // does not depend on how many null steps come between.  It races as
+        // in cuSMC when the step itself decides the outcome (full).
+        if (full)
             for (int c = 0; c < N; ++c)
                 if (null_[c]) {
                     const double r = rate_of(c);
@@ -672,7 +697,7 @@ private:
         // the step that crosses the bound: the left-out null racers' first
         // event after the bound (base rates) may come first; then the step
         // is a null step (w = -1)
-        if (!uniform && !full && *d_out < HUGE_VAL && !(t + *d_out < q_.time_bound)) {
+        if (!full && *d_out < HUGE_VAL && !(t + *d_out < q_.time_bound)) {
             bool any = false;
             dleaf_.assign(N, 0.0);
             for (int c = 0; c < N; ++c)
@@ -687,7 +712,7 @@ private:
                 if (Rd > 0.0) {
                     smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_DEAD, 0);
                     const double td = q_.time_bound + (-smc_log2(smc_u01(w.v[0], w.v[1])) / Rd);
-                    if (td < t + *d_out) { *d_out = td - t; *w_out = -1; won = false; }
+                    if (td < t + *d_out) { *d_out = td - t; *w_out = -1; won = false; uwon = false; }
                 }
             }
         }

@@ -394,3 +394,23 @@ def large_model(name: str) -> str:
     from pathlib import Path
     p = Path(__file__).resolve().parent / "golden" / "models" / "large" / (name + ".gz")
     return gzip.decompress(p.read_bytes()).decode()
+
+
+def guarded_uniform(n_dead: int = 0) -> str:
+    """U: l0 with invariant x <= 5 and one edge to Goal guarded by x >= 2,
+    beside n_dead components with no edge (default rate 1).  cuSMC redraws
+    the uniform delay on (0, 5 - x] at every step (after U's own null wins
+    with x < 2 and after every null step of the others); its hazard
+    1/(5 - tau) depends on the absolute time only, so for 2 <= T <= 5
+    Pr[<=T](<> Goal) = 1 - (5 - T)/3 = (T - 2)/3, whatever n_dead."""
+    dead = "".join(f'<template><name>D{k}</name><declaration></declaration>'
+                   f'<location id="idd{k}"><name>d</name></location><init ref="idd{k}"/></template>'
+                   for k in range(n_dead))
+    system = "system U" + "".join(f", D{k}" for k in range(n_dead)) + ";"
+    return HEAD + f"""<declaration>int dummy = 0;</declaration>
+<template><name>U</name><declaration>clock x;</declaration>
+<location id="id0"><name>l0</name><label kind="invariant">x &lt;= 5</label></location>
+<location id="id1"><name>Goal</name></location>
+<init ref="id0"/>
+<transition><source ref="id0"/><target ref="id1"/><label kind="guard">x &gt;= 2</label></transition>
+</template>{dead}<system>{system}</system></nta>"""

@@ -20,7 +20,7 @@ from cusmc_amd import generators as G
 
 import oracle
 
-from _models import (clock_rate_race, first_winner, name_locations, receiver_goal, receiver_goal_p, single_exponential,
+from _models import (clock_rate_race, first_winner, guarded_uniform, name_locations, receiver_goal, receiver_goal_p, single_exponential,
                      single_uniform, sir_exact, urgent_order, zeno)
 
 MODES = [oracle.CANONICAL, oracle.LITERAL]
@@ -280,3 +280,20 @@ def test_location_queries_on_broadcast_models_agree_in_law(models_dir):
                             ("UppaalBehaviorTest3.xml", "c2.g2", 20.0), ("UppaalBehaviorTest3.xml", "c1.f2", 5.0)):
         net = cs.ParsedNetwork.from_file(models_dir / fname, query)
         _agree(net, cs.reach_query(T), n)
+
+
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+@pytest.mark.parametrize("n_dead", [0, 3])
+@pytest.mark.parametrize("T", [3.0, 4.5])
+def test_redrawn_uniform_delay_closed_form(mode, n_dead, T):
+    """A uniform delay redrawn at every step (simulation.cu:461-521) keeps
+    the hazard 1/(W - tau) of one draw over the whole window: with the guard
+    x >= 2 and the window x <= 5, Pr = (T - 2)/3, with or without null
+    racers beside it.  The canonical sampling leaves the null racers out."""
+    net = cs.ParsedNetwork.from_string(guarded_uniform(n_dead), "")
+    n = 60000
+    res, ro = oracle.run(net, cs.reach_query(T), seed=16, n_runs=n, mode=mode, threads=8, per_run=("steps",))
+    p = (T - 2.0) / 3.0
+    assert _prob_ok(res.estimate, p, n), (res.estimate, p)
+    if mode == oracle.LITERAL and n_dead:
+        assert ro["steps"].mean() > 3.0  # the null racers' steps are there in cuSMC's sampling
