@@ -292,16 +292,12 @@ __device__ __forceinline__ long long edge_weight(int e, const double *vars, cons
     return w > 0 ? w : 0;
 }
 
-// Weighted choice among enabled edges (take_transition simulation.cu:43-79
-// and get_sum_edge_weight :9-24, with the weights of the enabled edges).
-// The draw is made only when there is a choice.
+// Weighted choice among the edges in mask (two or more) for the uniform u
+// (take_transition simulation.cu:43-79 and get_sum_edge_weight :9-24, with
+// the weights of the enabled edges).
 template <class M>
-__device__ __forceinline__ int pick_edge(uint64_t mask, int edge_first, uint64_t seed, uint64_t run,
-                                         uint32_t step, uint32_t purpose, uint32_t comp,
-                                         const double *vars, const M &m) {
-    if ((mask & (mask - 1)) == 0) return lowest_lane(mask);
-    const smc_u32x4 w = smc_draw(seed, run, step, purpose, comp);
-    const double u = smc_u01(w.v[0], w.v[1]);
+__device__ __forceinline__ int pick_weighted(uint64_t mask, int edge_first, double u, const double *vars,
+                                             const M &m) {
     long long W = 0;
     for (uint64_t r = mask; r; r &= r - 1) W += edge_weight(edge_first + lowest_lane(r), vars, m);
     const long long rnd = (long long)((double)W * u);
@@ -314,6 +310,27 @@ __device__ __forceinline__ int pick_edge(uint64_t mask, int edge_first, uint64_t
         temp -= w2;
     }
     return lowest_lane(mask);
+}
+
+// Weighted choice among the enabled edges in mask; the draw is made only
+// when there is a choice.
+template <class M>
+__device__ __forceinline__ int pick_edge(uint64_t mask, int edge_first, uint64_t seed, uint64_t run,
+                                         uint32_t step, uint32_t purpose, uint32_t comp,
+                                         const double *vars, const M &m) {
+    if ((mask & (mask - 1)) == 0) return lowest_lane(mask);
+    const smc_u32x4 w = smc_draw(seed, run, step, purpose, comp);
+    return pick_weighted(mask, edge_first, smc_u01(w.v[0], w.v[1]), vars, m);
+}
+
+// The same for component c as a broadcast receiver: its uniform is word
+// c & 3 of the receivers' shared draw c >> 2.
+template <class M>
+__device__ __forceinline__ int pick_receiver(uint64_t mask, int edge_first, uint64_t seed, uint64_t run,
+                                             uint32_t step, uint32_t c, const double *vars, const M &m) {
+    if ((mask & (mask - 1)) == 0) return lowest_lane(mask);
+    const smc_u32x4 w = smc_draw(seed, run, step, SMC_P_RECEIVER, smc_receiver_draw(c));
+    return pick_weighted(mask, edge_first, smc_receiver_u(w, c), vars, m);
 }
 
 // Rate sum, on the race tree, of the components that cannot fire before
@@ -461,13 +478,25 @@ __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int
     bool any_fired = false;
     double qsum = 0.0, qhi = -DBL_MAX, qlo = DBL_MAX;  // prefix sums of this lane's query additions
     bool qany = false;
+    // Receivers 4g .. 4g+3 share draw g: the lane's slots of one draw are
+    // G consecutive slots (C is a power of two), drawn once at the first of
+    // them when any of them has a choice to make.
+    constexpr int G = C < 4 ? C : 4;
+    smc_u32x4 rw = {{0u, 0u, 0u, 0u}};
 #pragma unroll
     for (int j = 0; j < C; ++j) {
         ej[j] = -1;
+        const uint32_t c = (uint32_t)(sl * C + j);
+        if (j % G == 0) {
+            bool choice = false;
+#pragma unroll
+            for (int k = j; k < j + G; ++k) choice |= (mk[k] & (mk[k] - 1)) != 0;
+            if (choice) rw = smc_draw(seed, run, step, SMC_P_RECEIVER, smc_receiver_draw(c));
+        }
         if (mk[j]) {
             const int first = (int)m.locs[loc[j]].edge_first;
-            const int e = first + pick_edge(mk[j], first, seed, run, step, SMC_P_RECEIVER,
-                                            (uint32_t)(sl * C + j), vars, m);
+            const int e = first + ((mk[j] & (mk[j] - 1)) ? pick_weighted(mk[j], first, smc_receiver_u(rw, c), vars, m)
+                                                         : lowest_lane(mk[j]));
             ej[j] = e;
             ++fired;
             any_fired = true;
@@ -563,7 +592,7 @@ __device__ __forceinline__ void fire_receivers_serial(uint64_t (&mk)[C], int (&l
                 int e_fired = -1;
                 if (lane == L) {
                     const int first = (int)m.locs[loc[0]].edge_first;
-                    e_fired = first + pick_edge(mk[0], first, seed, run, step, SMC_P_RECEIVER, (uint32_t)c, vars, m);
+                    e_fired = first + pick_receiver(mk[0], first, seed, run, step, (uint32_t)c, vars, m);
                     loc[0] = fire_edge(e_fired, vars, m, q, goal, dummy_chan);
                 }
                 wave_sync();

@@ -295,15 +295,22 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     // draws M (1 - (1 - u)^(1/a))
                     double umin = INF;
                     int unif_j = -1;
-                    for (uint32_t b = bmask; b; b &= b - 1) {
+                    for (uint32_t b = bmask; b;) {
                         const int j = __ffs(b) - 1;
-                        const smc_u32x4 rw = smc_draw(P.seed, run, step, SMC_P_UNIFORM, (uint32_t)(sl * C + j));
-                        const double u = smc_u01(rw.v[0], rw.v[1]);
-                        const double M = slot_get<C>(Mb, j);
-                        const int ti = theta ? tl.tix[slot_get<C>(loc, j)] : -1;
-                        const bool tilted = ti >= 0 && theta[ti] != 0.0f;
-                        const double du = tilted ? M * (1.0 - smc_exp2(smc_log2(1.0 - u) / tex[ti])) : M * u;
-                        if (du < umin) { umin = du; unif_j = j; }
+                        const uint32_t c = (uint32_t)(sl * C + j);
+                        const smc_u32x4 rw = smc_draw(P.seed, run, step, SMC_P_UNIFORM, smc_uniform_draw(c));
+                        // slot j, then slot j + 1 when it shares the draw
+                        const int nj = (C > 1 && !(c & 1u) && ((b >> (j + 1)) & 1u)) ? 2 : 1;
+                        for (int k = 0; k < nj; ++k) {
+                            const int jk = j + k;
+                            const double u = smc_uniform_u(rw, c + (uint32_t)k);
+                            const double M = slot_get<C>(Mb, jk);
+                            const int ti = theta ? tl.tix[slot_get<C>(loc, jk)] : -1;
+                            const bool tilted = ti >= 0 && theta[ti] != 0.0f;
+                            const double du = tilted ? M * (1.0 - smc_exp2(smc_log2(1.0 - u) / tex[ti])) : M * u;
+                            if (du < umin) { umin = du; unif_j = jk; }
+                            b &= b - 1;
+                        }
                     }
                     // tilted race: the run kernel's tree over the tilted rates
                     double tr[2 * C];

@@ -242,14 +242,22 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                         w = (Lz - seg_base) * C + __shfl(zero_j, Lz);
                         d = 0.0;
                     } else {
-                        // uniform delays under clock bounds, lowest slot first on ties
+                        // uniform delays under clock bounds, lowest slot first on ties;
+                        // slots 2i and 2i+1 of a lane share one draw
                         double umin = INF;
                         int unif_j = -1;
-                        for (uint32_t b = bmask; b; b &= b - 1) {
+                        for (uint32_t b = bmask; b;) {
                             const int j = __ffs(b) - 1;
-                            const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_UNIFORM, (uint32_t)(sl * C + j));
-                            const double du = slot_get<C>(Mb, j) * smc_u01(rw.v[0], rw.v[1]);
+                            const uint32_t c = (uint32_t)(sl * C + j);
+                            const smc_u32x4 rw = smc_draw(p.seed, run, step, SMC_P_UNIFORM, smc_uniform_draw(c));
+                            const double du = slot_get<C>(Mb, j) * smc_uniform_u(rw, c);
                             if (du < umin) { umin = du; unif_j = j; }
+                            b &= b - 1;
+                            if (C > 1 && !(c & 1u) && ((b >> (j + 1)) & 1u)) {
+                                const double du1 = slot_get<C>(Mb, j + 1) * smc_uniform_u(rw, c + 1);
+                                if (du1 < umin) { umin = du1; unif_j = j + 1; }
+                                b &= b - 1;
+                            }
                         }
                         // pairwise tree over the lane's C leaves (heap order,
                         // root tr[1]); only the root lives through the butterfly

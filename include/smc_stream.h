@@ -68,16 +68,23 @@ SMC_HD smc_u32x4 smc_philox4x32_10(smc_u32x4 ctr, uint32_t k0, uint32_t k1) {
 /* --------------------------------------------------------- stream layout */
 
 /* Purposes of a draw.  A draw is addressed by
- *   counter = (step, purpose << 24 | component, run_lo, run_hi),
+ *   counter = (step, purpose << 24 | index, run_lo, run_hi),
  *   key     = (seed_lo, seed_hi).
- * Within one step a component makes at most one draw per purpose, so no two
- * draws of one run ever share a counter. */
+ * Within one step each (purpose, index) is drawn at most once, and each of
+ * its words serves one component, so no two uses of one run share bits.
+ * Neighbouring components share a draw where one needs fewer than 128 bits:
+ * the uniform delays of components 2g and 2g+1 are draw g of SMC_P_UNIFORM
+ * (words 0-1 and 2-3), the receiver choices of components 4g .. 4g+3 are
+ * draw g of SMC_P_RECEIVER (word c & 3), so a lane holding several
+ * components makes one call where it would make two or four. */
 enum smc_purpose {
     SMC_P_RACE = 0,     /* per run and step: exponential race (words 0-1: the
                            race delay, words 2-3: which exponential component) */
-    SMC_P_UNIFORM = 1,  /* per component: delay under a clock upper bound     */
+    SMC_P_UNIFORM = 1,  /* per pair of components: delay under a clock upper
+                           bound (53 bits each)                              */
     SMC_P_WINNER = 2,   /* per component: weighted edge choice of the winner  */
-    SMC_P_RECEIVER = 3, /* per component: weighted choice of a broadcast receiver */
+    SMC_P_RECEIVER = 3, /* per four components: weighted choice of a broadcast
+                           receiver (32 bits each)                           */
     SMC_P_TILT = 4,     /* importance sampling: spare stream                  */
     SMC_P_DEAD = 5      /* per run and step: first null event after the time
                            bound of the components that cannot fire        */
@@ -97,6 +104,26 @@ SMC_HD smc_u32x4 smc_draw(uint64_t seed, uint64_t run, uint32_t step,
 SMC_HD double smc_u01(uint32_t hi, uint32_t lo) {
     uint64_t k = ((uint64_t)hi << 21) | (uint64_t)(lo >> 11);
     return (double)(k + 1u) * 1.1102230246251565404236316680908203125e-16;
+}
+
+/* Uniform double on (0, 1] from one word: u = (k + 1) * 2^-32 (exact).  A
+ * weighted choice among edges of integer weights takes floor(W u), so each
+ * edge's probability is off its weight share by less than 2^-32 (cuSMC's
+ * curand_uniform draws carry 24 bits). */
+SMC_HD double smc_u01_32(uint32_t k) {
+    return ((double)k + 1.0) * 2.3283064365386962890625e-10;
+}
+
+/* The draw and words of component c's uniform delay and receiver choice. */
+SMC_HD uint32_t smc_uniform_draw(uint32_t c) { return c >> 1; }
+SMC_HD double smc_uniform_u(smc_u32x4 w, uint32_t c) {
+    return (c & 1u) ? smc_u01(w.v[2], w.v[3]) : smc_u01(w.v[0], w.v[1]);
+}
+SMC_HD uint32_t smc_receiver_draw(uint32_t c) { return c >> 2; }
+SMC_HD double smc_receiver_u(smc_u32x4 w, uint32_t c) {
+    const uint32_t a = (c & 1u) ? w.v[1] : w.v[0];
+    const uint32_t b = (c & 1u) ? w.v[3] : w.v[2];
+    return smc_u01_32((c & 2u) ? b : a);
 }
 
 /* ------------------------------------------------------------- log2 */

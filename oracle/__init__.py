@@ -90,6 +90,10 @@ def lib() -> C.CDLL:
         l.oracle_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         l.oracle_u01.restype = C.c_double
         l.oracle_u01.argtypes = [C.c_uint32, C.c_uint32]
+        l.oracle_shared_u.restype = C.c_double
+        l.oracle_shared_u.argtypes = [C.POINTER(C.c_uint32), C.c_uint32, C.c_int]
+        l.oracle_shared_draw.restype = C.c_uint32
+        l.oracle_shared_draw.argtypes = [C.c_uint32, C.c_int]
         l.oracle_log2.restype = C.c_double
         l.oracle_log2.argtypes = [C.c_double]
         l.oracle_pow.restype = C.c_double

@@ -323,10 +323,16 @@ public:
                     recv_[c].clear();
                     if (c != sender) enabled(loc_[c], chan, &recv_[c]);
                 }
+                int group = -1;  // receivers 4g .. 4g+3 share one draw
                 for (int c = 0; c < N; ++c) {
                     if (recv_[c].empty()) continue;
-                    smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_RECEIVER, (uint32_t)c);
-                    int e = pick(recv_[c], smc_u01(w.v[0], w.v[1]));
+                    const uint32_t g = smc_receiver_draw((uint32_t)c);
+                    if (recv_[c].size() > 1 && (int)g != group) {
+                        cost(COST_PHILOX);
+                        group = (int)g;
+                    }
+                    smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_RECEIVER, g);
+                    int e = pick(recv_[c], smc_receiver_u(w, (uint32_t)c));
                     int dummy = 0;
                     fire(c, e, &goal, &dummy);
                     log(c, e);
@@ -391,6 +397,7 @@ public:
             if (w >= 0) enabled(loc_[w], 0, &en);
             if (!en.empty()) {
                 smc_u32x4 r = smc_draw(seed_, run, step, SMC_P_WINNER, (uint32_t)w);
+                if (en.size() > 1) cost(COST_PHILOX);
                 int e = pick(en, smc_u01(r.v[0], r.v[1]));
                 int nc = 0;
                 fire(w, e, &goal, &nc);
@@ -549,7 +556,7 @@ private:
 
     int pick(const std::vector<int> &en, double u) const {
         if (en.size() == 1) return en[0];
-        cost(COST_PHILOX + COST_U01 + 4 * COST_OP * (double)en.size());
+        cost(COST_U01 + 4 * COST_OP * (double)en.size());  // the caller charges the draw
         long long W = 0;
         for (int e : en) W += weight(e);
         const long long rnd = (long long)((double)W * u);
@@ -622,7 +629,7 @@ private:
                         bool *won_out) {
         const int N = n_.n_components;
         const float *theta = is_ ? is_->theta : nullptr;
-        int zero = -1, wu = -1;
+        int zero = -1, wu = -1, ugroup = -1;
         double du = HUGE_VAL;
         cost(2 * COST_OP * N);  // delay class and leaf of every component
         bool uniform = false;
@@ -635,9 +642,14 @@ private:
             double M;
             if (bound_of(c, &M)) {
                 if (M == 0.0) { if (zero < 0) zero = c; continue; }
-                cost(COST_PHILOX + COST_U01 + 3 * COST_OP);
-                smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_UNIFORM, (uint32_t)c);
-                const double u = smc_u01(w.v[0], w.v[1]);
+                const uint32_t g = smc_uniform_draw((uint32_t)c);  // components 2g, 2g+1 share a draw
+                if ((int)g != ugroup) {
+                    cost(COST_PHILOX);
+                    ugroup = (int)g;
+                }
+                cost(COST_U01 + 3 * COST_OP);
+                smc_u32x4 w = smc_draw(seed_, run, step, SMC_P_UNIFORM, g);
+                const double u = smc_uniform_u(w, (uint32_t)c);
                 const double th = theta ? (double)theta[loc_[c]] : 0.0;
                 const double d = th == 0.0 ? M * u : M * (1.0 - smc_exp2(smc_log2(1.0 - u) / smc_exp2(th)));
                 um_[c] = M;
@@ -930,6 +942,14 @@ void oracle_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
     for (int i = 0; i < 4; ++i) out[i] = o.v[i];
 }
 double oracle_u01(uint32_t hi, uint32_t lo) { return smc_u01(hi, lo); }
+double oracle_shared_u(const uint32_t *w, uint32_t c, int receiver) {
+    smc_u32x4 x;
+    for (int i = 0; i < 4; ++i) x.v[i] = w[i];
+    return receiver ? smc_receiver_u(x, c) : smc_uniform_u(x, c);
+}
+uint32_t oracle_shared_draw(uint32_t c, int receiver) {
+    return receiver ? smc_receiver_draw(c) : smc_uniform_draw(c);
+}
 double oracle_log2(double x) { return smc_log2(x); }
 double oracle_pow(double a, double b) { return smc_pow(a, b); }
 double oracle_eval_expr(const smc_network *net, int e, const double *vars) { return eval(*net, e, vars); }

@@ -95,6 +95,10 @@ double oracle_tree_pick(const double *rates, int n, double v, int *chosen);
 /* Stream primitives and expression evaluation, exposed for tests. */
 void oracle_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 double oracle_u01(uint32_t hi, uint32_t lo);
+/* Component c's uniform from its shared draw w[4] (receiver != 0: a receiver
+ * choice, else a uniform delay), and the index of that draw. */
+double oracle_shared_u(const uint32_t *w, uint32_t c, int receiver);
+uint32_t oracle_shared_draw(uint32_t c, int receiver);
 double oracle_log2(double x);
 double oracle_pow(double a, double b);
 double oracle_eval_expr(const smc_network *net, int e, const double *vars);

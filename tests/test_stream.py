@@ -51,6 +51,27 @@ def test_u01_range_and_extremes():
         assert 0.0 < u <= 1.0
 
 
+def test_shared_draw_words():
+    """Uniform delays: components 2g, 2g+1 take words 0-1 and 2-3 of draw g;
+    receiver choices: components 4g..4g+3 take word c mod 4 of draw g, each
+    word a 32-bit uniform (k + 1) 2^-32 on (0, 1]."""
+    import ctypes as C
+    L = oracle.lib()
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        w = [int(x) for x in rng.integers(0, 2 ** 32, size=4, dtype=np.uint64)]
+        arr = (C.c_uint32 * 4)(*w)
+        for c in range(12):
+            assert L.oracle_shared_draw(c, 0) == c // 2
+            assert L.oracle_shared_draw(c, 1) == c // 4
+            hi, lo = w[2 * (c % 2)], w[2 * (c % 2) + 1]
+            assert L.oracle_shared_u(arr, c, 0) == L.oracle_u01(hi, lo)
+            assert L.oracle_shared_u(arr, c, 1) == (w[c % 4] + 1) * 2.0 ** -32
+    ext = (C.c_uint32 * 4)(0, 0xFFFFFFFF, 5, 6)
+    assert L.oracle_shared_u(ext, 0, 1) == 2.0 ** -32
+    assert L.oracle_shared_u(ext, 1, 1) == 1.0
+
+
 def test_log2_exact_at_powers_of_two():
     L = oracle.lib()
     for k in range(-1074 + 52, 1000, 37):
