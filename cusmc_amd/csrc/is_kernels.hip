@@ -197,21 +197,27 @@ __global__ void fold_splits_kernel(const float *part, int splits, size_t n, floa
 // ------------------------------------------------------------ GEMM
 //
 // C[M x N] (+)= op(A)[M x K] . op(B)[K x N], bf16 inputs, fp32 accumulation
-// on MFMA 16x16x16.  A workgroup of four waves computes a 64 x 64 tile; each
-// wave a 32 x 32 quarter as 2 x 2 MFMA blocks; K advances 16 at a time
-// through LDS.  Operand layout of v_mfma_f32_16x16x16_bf16: lane l holds
-// A[l % 16][4 (l / 16) .. +3] and B[4 (l / 16) .. +3][l % 16]; the result
-// lane l holds rows 4 (l / 16) .. +3 of column l % 16.  M, N are multiples
-// of 64 and K of 16 (the host pads every buffer).  Each output element sums
-// K in a fixed order, so a row's result does not depend on the other rows.
+// on gfx950's MFMA 16x16x32.  A workgroup of four waves computes a 64 x 64
+// tile; each wave a 32 x 32 quarter as 2 x 2 MFMA blocks; K advances 32 at a
+// time through LDS.  Operand layout of v_mfma_f32_16x16x32_bf16: lane l
+// holds A[l % 16][8 (l / 16) .. +7] and B[8 (l / 16) .. +7][l % 16] (eight
+// bf16, one 16-byte LDS read each: A is staged [m][k], B transposed [n][k]);
+// the result lane l holds rows 4 (l / 16) .. +3 of column l % 16.  M, N are
+// multiples of 64, K of 32, leading dimensions and base addresses of 8
+// elements (the host pads every buffer).  Each output element sums K in a
+// fixed order, so a row's result does not depend on the other rows.
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Staged tile rows: 32 bf16 of K plus 8 of padding, so 16-byte fragment
+// reads stay aligned and eight consecutive rows start in distinct banks.
+constexpr int kGemmKP = 32 + 8;
 
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[64][16 + 4];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[16][64 + 4];
+    __shared__ __attribute__((aligned(16))) uint16_t As[64][kGemmKP];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[64][kGemmKP];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, l = tid & 63;
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
@@ -224,52 +230,50 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     const int kz = (int)blockIdx.z * g.kchunk;
     const int kend = min(g.K, kz + g.kchunk);
     float *const Cout = g.C ? g.C + (size_t)blockIdx.z * g.zstride : nullptr;
-    // A (64 x 16) and B (16 x 64) tiles, 1024 elements each: every thread
-    // moves four consecutive elements of each with one 8-byte load; the
-    // next tile's loads are issued before the current tile's MFMAs
-    const int ta_k = tid >> 4, ta_i = 4 * (tid & 15);  // A stored [K][M]
-    const int na_i = tid >> 2, na_k = 4 * (tid & 3);   // A stored [M][K]
-    const int tb_j = tid >> 2, tb_k = 4 * (tid & 3);   // B stored [N][K]
-    const int nb_k = tid >> 4, nb_j = 4 * (tid & 15);  // B stored [K][N]
-    auto load_a = [&](int k0) -> uint2 {
-        return TA ? *(const uint2 *)(g.A + (size_t)(k0 + ta_k) * g.lda + (m0 + ta_i))
-                  : *(const uint2 *)(g.A + (size_t)(m0 + na_i) * g.lda + (k0 + na_k));
+    // A and B tiles of 64 x 32, 2048 elements each: every thread moves eight
+    // consecutive elements of each with one 16-byte load; the next tile's
+    // loads are issued before the current tile's MFMAs
+    const int ta_k = tid >> 3, ta_i = 8 * (tid & 7);  // A stored [K][M]
+    const int na_i = tid >> 2, na_k = 8 * (tid & 3);  // A stored [M][K]
+    const int tb_j = tid >> 2, tb_k = 8 * (tid & 3);  // B stored [N][K]
+    const int nb_k = tid >> 3, nb_j = 8 * (tid & 7);  // B stored [K][N]
+    auto load_a = [&](int k0) -> uint4 {
+        return TA ? *(const uint4 *)(g.A + (size_t)(k0 + ta_k) * g.lda + (m0 + ta_i))
+                  : *(const uint4 *)(g.A + (size_t)(m0 + na_i) * g.lda + (k0 + na_k));
     };
-    auto load_b = [&](int k0) -> uint2 {
-        return TB ? *(const uint2 *)(g.B + (size_t)(n0 + tb_j) * g.ldb + (k0 + tb_k))
-                  : *(const uint2 *)(g.B + (size_t)(k0 + nb_k) * g.ldb + (n0 + nb_j));
+    auto load_b = [&](int k0) -> uint4 {
+        return TB ? *(const uint4 *)(g.B + (size_t)(n0 + tb_j) * g.ldb + (k0 + tb_k))
+                  : *(const uint4 *)(g.B + (size_t)(k0 + nb_k) * g.ldb + (n0 + nb_j));
     };
-    uint2 ra = kz < kend ? load_a(kz) : uint2{0u, 0u};
-    uint2 rb = kz < kend ? load_b(kz) : uint2{0u, 0u};
-    for (int k0 = kz; k0 < kend; k0 += 16) {
-        if (TA) {
-            As[ta_i + 0][ta_k] = (uint16_t)ra.x; As[ta_i + 1][ta_k] = (uint16_t)(ra.x >> 16);
-            As[ta_i + 2][ta_k] = (uint16_t)ra.y; As[ta_i + 3][ta_k] = (uint16_t)(ra.y >> 16);
-        } else {
-            *(uint2 *)&As[na_i][na_k] = ra;
+    // eight elements of one k, consecutive in m (or n), into a [m][k] tile
+    auto scatter = [](uint16_t (*T)[kGemmKP], int i0, int k, uint4 v) {
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            T[i0 + 2 * q][k] = (uint16_t)w[q];
+            T[i0 + 2 * q + 1][k] = (uint16_t)(w[q] >> 16);
         }
-        if (TB) {
-            Bs[tb_k + 0][tb_j] = (uint16_t)rb.x; Bs[tb_k + 1][tb_j] = (uint16_t)(rb.x >> 16);
-            Bs[tb_k + 2][tb_j] = (uint16_t)rb.y; Bs[tb_k + 3][tb_j] = (uint16_t)(rb.y >> 16);
-        } else {
-            *(uint2 *)&Bs[nb_k][nb_j] = rb;
-        }
+    };
+    uint4 ra = kz < kend ? load_a(kz) : uint4{0u, 0u, 0u, 0u};
+    uint4 rb = kz < kend ? load_b(kz) : uint4{0u, 0u, 0u, 0u};
+    for (int k0 = kz; k0 < kend; k0 += 32) {
+        if (TA) scatter(As, ta_i, ta_k, ra);
+        else *(uint4 *)&As[na_i][na_k] = ra;
+        if (TB) *(uint4 *)&Bs[tb_j][tb_k] = rb;
+        else scatter(Bs, nb_j, nb_k, rb);
         __syncthreads();
-        if (k0 + 16 < kend) {
-            ra = load_a(k0 + 16);
-            rb = load_b(k0 + 16);
+        if (k0 + 32 < kend) {
+            ra = load_a(k0 + 32);
+            rb = load_b(k0 + 32);
         }
+        const int kb = 8 * (l >> 4);
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            const int row = wr + a * 16 + (l & 15), kb = 4 * (l >> 4);
-            const bf16x4 fa = *(const bf16x4 *)&As[row][kb];
+            const bf16x8 fa = *(const bf16x8 *)&As[wr + a * 16 + (l & 15)][kb];
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                bf16x4 fb;
-                const int col = wc + b * 16 + (l & 15);
-                fb[0] = (short)Bs[kb][col]; fb[1] = (short)Bs[kb + 1][col];
-                fb[2] = (short)Bs[kb + 2][col]; fb[3] = (short)Bs[kb + 3][col];
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fb, acc[a][b], 0, 0, 0);
+                const bf16x8 fb = *(const bf16x8 *)&Bs[wc + b * 16 + (l & 15)][kb];
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc[a][b], 0, 0, 0);
             }
         }
         __syncthreads();
@@ -321,29 +325,30 @@ __global__ void adam_kernel(float *w, const float *gsum, float *m1, float *m2, s
 // ------------------------------------------------------------ fused forward
 //
 // Persistent workgroups of eight waves.  The workgroup stages the three
-// weight matrices transposed ([n][k], rows padded by 4 bf16 against bank
-// conflicts) and the biases in LDS once; each wave then takes 16-row tiles:
-// layer 1 reads its A fragments straight from the feature rows, layers 2
-// and 3 from the wave's activation tile in LDS.  MFMA 16x16x16 bf16 with
-// fp32 accumulation, K in steps of 16 from 0 upwards, as gemm_kernel.
+// weight matrices transposed ([n][k], rows padded by 8 bf16: aligned 16-byte
+// fragment reads, distinct banks for eight consecutive rows) and the biases
+// in LDS once; each wave then takes 16-row tiles: layer 1 reads its A
+// fragments straight from the feature rows, layers 2 and 3 from the wave's
+// activation tile in LDS.  MFMA 16x16x32 bf16 with fp32 accumulation, K in
+// steps of 32 from 0 upwards, as gemm_kernel, so both give the same bits.
 
 // Waves per workgroup: two per SIMD share one copy of the weights, so one
 // wave's memory waits overlap the other's MFMAs.
 constexpr int kMlpWaves = 8;
 
 __host__ __device__ constexpr size_t mlp_image_size(int fpad, int H, int g16) {
-    return 2 * ((size_t)H * (fpad + 4) + (size_t)H * (H + 4) + (size_t)g16 * (H + 4)) + 4 * (2 * (size_t)H + g16);
+    return 2 * ((size_t)H * (fpad + 8) + (size_t)H * (H + 8) + (size_t)g16 * (H + 8)) + 4 * (2 * (size_t)H + g16);
 }
 
 __host__ __device__ constexpr size_t mlp_lds_bytes(int fpad, int H, int g16) {
-    return mlp_image_size(fpad, H, g16) + 2 * (size_t)kMlpWaves * 16 * (H + 4);
+    return mlp_image_size(fpad, H, g16) + 2 * (size_t)kMlpWaves * 16 * (H + 8);
 }
 
 // The packed image, one thread per element: bf16 part, then fp32 biases.
 __global__ void mlp_pack_kernel(const float *w, size_t o_w1, size_t o_b1, size_t o_w2, size_t o_b2, size_t o_w3,
                                 size_t o_b3, int fpad, int H, int gpad, int g16, uint16_t *img) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t KP1 = fpad + 4, KP = H + 4;
+    const size_t KP1 = fpad + 8, KP = H + 8;
     const size_t n1 = (size_t)H * KP1, n2 = (size_t)H * KP, n3 = (size_t)g16 * KP;
     if (i < n1) {
         const size_t n = i / KP1, k = i % KP1;
@@ -363,13 +368,13 @@ __global__ void mlp_pack_kernel(const float *w, size_t o_w1, size_t o_b1, size_t
     }
 }
 
-constexpr int kMlpMaxK = 32;  // layer-1 k-steps held in registers: fpad <= 512
+constexpr int kMlpMaxK = 16;  // layer-1 k-steps of 32 held in registers: fpad <= 512
 
 template <int H>
 __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpFwdArgs a) {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
-    constexpr int NT = H / 16, KP = H + 4;
-    const int KP1 = a.fpad + 4, g16 = a.g16;
+    constexpr int NT = H / 16, KP = H + 8;
+    const int KP1 = a.fpad + 8, g16 = a.g16;
     uint16_t *w1t = smem;
     uint16_t *w2t = w1t + (size_t)H * KP1;
     uint16_t *w3t = w2t + (size_t)H * KP;
@@ -383,7 +388,7 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpFwdArgs 
                 (uint32_t)(mlp_image_size(a.fpad, H, g16) / 16));
     __syncthreads();
 
-    const int ar = l & 15, kq = 4 * (l >> 4);  // A/B fragments: row (column) and first k
+    const int ar = l & 15, kq = 8 * (l >> 4);  // A/B fragments: row (column) and first k
     const int cr = 4 * (l >> 4), cc = l & 15;  // C fragment: rows cr..cr+3, column cc
     const int tiles = a.rows / 16;
     for (int t = blockIdx.x * kMlpWaves + wave; t < tiles; t += gridDim.x * kMlpWaves) {
@@ -394,18 +399,18 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpFwdArgs 
         // layer 1: the tile's A fragments are loaded first, all at once (the
         // loads overlap instead of each k-step waiting on memory)
         const uint16_t *xrow = a.X + (size_t)(r0 + ar) * a.fpad + kq;
-        const int KT = a.fpad / 16;
-        bf16x4 xa[kMlpMaxK];
+        const int KT = a.fpad / 32;
+        bf16x8 xa[kMlpMaxK];
 #pragma unroll
         for (int kt = 0; kt < kMlpMaxK; ++kt)
-            if (kt < KT) xa[kt] = *(const bf16x4 *)(xrow + kt * 16);
+            if (kt < KT) xa[kt] = *(const bf16x8 *)(xrow + kt * 32);
 #pragma unroll
         for (int kt = 0; kt < kMlpMaxK; ++kt)
             if (kt < KT) {
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
-                    const bf16x4 fb = *(const bf16x4 *)(w1t + (size_t)(nt * 16 + ar) * KP1 + kt * 16 + kq);
-                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xa[kt], fb, acc[nt], 0, 0, 0);
+                    const bf16x8 fb = *(const bf16x8 *)(w1t + (size_t)(nt * 16 + ar) * KP1 + kt * 32 + kq);
+                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa[kt], fb, acc[nt], 0, 0, 0);
                 }
             }
 #pragma unroll
@@ -424,12 +429,12 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpFwdArgs 
         wave_sync();
         // layer 2
 #pragma unroll 2
-        for (int k0 = 0; k0 < H; k0 += 16) {
-            const bf16x4 fa = *(const bf16x4 *)(hb + ar * KP + k0 + kq);
+        for (int k0 = 0; k0 < H; k0 += 32) {
+            const bf16x8 fa = *(const bf16x8 *)(hb + ar * KP + k0 + kq);
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-                const bf16x4 fb = *(const bf16x4 *)(w2t + (nt * 16 + ar) * KP + k0 + kq);
-                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fb, acc[nt], 0, 0, 0);
+                const bf16x8 fb = *(const bf16x8 *)(w2t + (nt * 16 + ar) * KP + k0 + kq);
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc[nt], 0, 0, 0);
             }
         }
         wave_sync();
@@ -451,13 +456,13 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpFwdArgs 
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) o[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 2
-        for (int k0 = 0; k0 < H; k0 += 16) {
-            const bf16x4 fa = *(const bf16x4 *)(hb + ar * KP + k0 + kq);
+        for (int k0 = 0; k0 < H; k0 += 32) {
+            const bf16x8 fa = *(const bf16x8 *)(hb + ar * KP + k0 + kq);
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt)
                 if (nt * 16 < g16) {
-                    const bf16x4 fb = *(const bf16x4 *)(w3t + (nt * 16 + ar) * KP + k0 + kq);
-                    o[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fb, o[nt], 0, 0, 0);
+                    const bf16x8 fb = *(const bf16x8 *)(w3t + (nt * 16 + ar) * KP + k0 + kq);
+                    o[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, o[nt], 0, 0, 0);
                 }
         }
 #pragma unroll
@@ -527,7 +532,7 @@ static int max_lds_per_block() {
 }
 
 bool mlp_forward_supported(int fpad, int H, int g16, float clamp_max) {
-    if (!(H == 64 || H == 128 || H == 256) || fpad <= 0 || fpad % 16 || fpad > 16 * kMlpMaxK || g16 <= 0 ||
+    if (!(H == 64 || H == 128 || H == 256) || fpad <= 0 || fpad % 32 || fpad > 32 * kMlpMaxK || g16 <= 0 ||
         g16 % 16 || g16 > 64 ||
         !(clamp_max > 0.f))
         return false;
@@ -538,7 +543,7 @@ size_t mlp_image_bytes(int fpad, int H, int g16) { return mlp_image_size(fpad, H
 
 hipError_t mlp_pack(const float *w, size_t o_w1, size_t o_b1, size_t o_w2, size_t o_b2, size_t o_w3, size_t o_b3,
                     int fpad, int H, int gpad, int g16, void *image, hipStream_t s) {
-    const size_t n = (size_t)H * (fpad + 4) + (size_t)H * (H + 4) + (size_t)g16 * (H + 4) + 2 * (size_t)H + g16;
+    const size_t n = (size_t)H * (fpad + 8) + (size_t)H * (H + 8) + (size_t)g16 * (H + 8) + 2 * (size_t)H + g16;
     hipLaunchKernelGGL(mlp_pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w, o_w1, o_b1, o_w2, o_b2,
                        o_w3, o_b3, fpad, H, gpad, g16, (uint16_t *)image);
     return hipGetLastError();
@@ -566,7 +571,8 @@ hipError_t mlp_forward(const MlpFwdArgs &a, hipStream_t s) {
 }
 
 hipError_t gemm(const GemmArgs &g0, bool ta, bool tb, hipStream_t s) {
-    if (g0.M % 64 || g0.N % 64 || g0.K % 16 || g0.M <= 0 || g0.N <= 0 || g0.K <= 0) return hipErrorInvalidValue;
+    if (g0.M % 64 || g0.N % 64 || g0.K % 32 || g0.M <= 0 || g0.N <= 0 || g0.K <= 0 || g0.lda % 8 || g0.ldb % 8)
+        return hipErrorInvalidValue;
     GemmArgs g = g0;
     g.kchunk = g.K;
     g.zstride = 0;
@@ -574,11 +580,13 @@ hipError_t gemm(const GemmArgs &g0, bool ta, bool tb, hipStream_t s) {
 }
 
 hipError_t gemm_splitk(const GemmArgs &g0, bool ta, bool tb, int splits, float *work, hipStream_t s) {
-    if (g0.M % 64 || g0.N % 64 || g0.K % 16 || g0.M <= 0 || g0.N <= 0 || g0.K <= 0 || !g0.C) return hipErrorInvalidValue;
+    if (g0.M % 64 || g0.N % 64 || g0.K % 32 || g0.M <= 0 || g0.N <= 0 || g0.K <= 0 || g0.lda % 8 || g0.ldb % 8 ||
+        !g0.C)
+        return hipErrorInvalidValue;
     if (g0.bias || g0.relu || g0.mask || g0.clamp_max > 0.f || g0.Cb || g0.ldc != g0.N) return hipErrorInvalidValue;
     if (splits <= 1) return gemm(g0, ta, tb, s);
     int kchunk = (g0.K + splits - 1) / splits;
-    kchunk = (kchunk + 15) / 16 * 16;
+    kchunk = (kchunk + 31) / 32 * 32;
     const int z = (g0.K + kchunk - 1) / kchunk;
     GemmArgs g = g0;
     g.C = work;

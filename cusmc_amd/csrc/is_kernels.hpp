@@ -135,11 +135,13 @@ hipError_t expand_groups(const float *tg, int gpad, const int32_t *group, int n_
 // into ggb unless it is null).
 hipError_t group_sum(const float *gl, int lpad, const int32_t *member_first, const int32_t *members, int n_groups,
                      int gpad, int rows, float *gg, uint16_t *ggb, hipStream_t s);
+// C (+)= op(A) op(B) on MFMA 16x16x32: M, N multiples of 64, K of 32, lda
+// and ldb of 8, 16-byte aligned operands; hipErrorInvalidValue otherwise.
 hipError_t gemm(const GemmArgs &g, bool ta, bool tb, hipStream_t s);
 // Whether mlp_forward handles these sizes (H 64/128/256, weights in LDS).
 bool mlp_forward_supported(int fpad, int H, int g16, float clamp_max);
 // Bytes of the packed weight image: W1, W2, W3 transposed to [n][k] bf16
-// (rows padded by 4), then b1, b2, b3 (first g16) in fp32.
+// (rows padded by 8), then b1, b2, b3 (first g16) in fp32.
 size_t mlp_image_bytes(int fpad, int H, int g16);
 // Pack it from the flat fp32 parameters (offsets of W1, b1, W2, b2, W3, b3);
 // bf16 rounding as to_bf16, so the fused pass multiplies the same values.
