@@ -352,8 +352,8 @@ def main() -> int:
     ap.add_argument("--epochs", type=int, default=6, help="is: epochs per cross-entropy round")
     ap.add_argument("--lr", type=float, default=1e-2, help="is: Adam step")
     ap.add_argument("--lanes", type=int, default=0, help="fixed lanes per run (default: autotuned)")
-    ap.add_argument("--occupancy", type=int, default=0, choices=[0, 2],
-                    help="with --lanes: register policy (0 unrestricted, 2 two waves per SIMD)")
+    ap.add_argument("--occupancy", type=int, default=0, choices=[0, 3],
+                    help="with --lanes: register policy (0 unrestricted, 3 three waves per SIMD)")
     ap.add_argument("--tables", type=int, default=0, choices=[0, 1],
                     help="with --lanes: model tables in global memory (0) or staged in LDS (1)")
     ap.add_argument("--no-cpu", action="store_true")

@@ -35,7 +35,7 @@ namespace {
 // Launch shape for one group of instances.
 struct Plan {
     int S = 1, C = 1, R = 64, stride = 1;
-    int occ = 0;  // run-kernel build: 0 registers unrestricted, 2 two waves per SIMD
+    int occ = 0;  // run-kernel build: 0 registers unrestricted, 3 three waves per SIMD
     bool tab = false;     // tables staged in LDS
     int group_waves = 1;  // waves per workgroup
     uint32_t table_bytes = 0, wave_lds = 0;
@@ -50,7 +50,7 @@ constexpr size_t kLdsPerCu = 160 * 1024;
 Plan plan_launch(int P, int max_vars, int lanes, int occ, int tables, uint32_t table_bytes, int waves_req,
                  int waves_per_cu, uint64_t total_runs, uint64_t max_runs_per_inst, bool trace, int cu_count) {
     Plan pl;
-    if (occ != 0 && occ != 2) throw std::runtime_error("occupancy must be 0 or 2");
+    if (occ != 0 && occ != 3) throw std::runtime_error("occupancy must be 0 or 3");
     if (tables != 0 && tables != 1) throw std::runtime_error("tables must be 0 or 1");
     pl.occ = occ;
     int S = lanes ? lanes : (P < 64 ? P : 64);
@@ -542,7 +542,7 @@ int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sampl
         std::vector<smc_launch> cands;
         for (int S = P < 64 ? P : 64; S >= 1; S >>= 1) {
             if (P / S > 16) break;
-            for (int occ : {0, 2})
+            for (int occ : {0, 3})
                 for (int tab : {0, 1}) {
                     cands.push_back(smc_launch{S, 0, 0, occ, tab});
                     int blocks = 0;
@@ -746,7 +746,7 @@ int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_bat
             uint64_t sample = 0;
             for (auto &it : b->items) sample = std::max<uint64_t>(sample, std::min<uint64_t>(it.n_runs, 2048));
             for (int C = Cmin; C <= 16; C <<= 1)
-                for (int occ : {0, 2}) {
+                for (int occ : {0, 3}) {
                     BatchShape probe = batch_shape(b.get(), C, 1, sample);
                     const int waves = batch_waves(C, occ, probe.wave_lds, 0, cu);
                     BatchShape sh = batch_shape(b.get(), C, waves, sample);

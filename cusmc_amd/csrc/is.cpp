@@ -557,7 +557,7 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         if (m->has_tuned && m->tuned.lanes_per_run > 0 && m->tuned.lanes_per_run <= s->S &&
             P / m->tuned.lanes_per_run <= 8) {
             s->S = m->tuned.lanes_per_run;
-            s->occ = m->tuned.occupancy == 2 ? 2 : 0;
+            s->occ = m->tuned.occupancy == 3 ? 3 : 0;
         }
         // SMC_IS_LANES=<power of two> overrides the lanes per run (diagnosis:
         // the sampler's best packing may differ from the plain simulator's)
@@ -635,7 +635,7 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
             s->tab = want && m->table_bytes > 0 && need <= 160 * 1024;
             s->lds = s->tab ? need : shared + s->wave_lds;
             if (s->lds > 160 * 1024) throw std::runtime_error("limit: the advance kernel's LDS exceeds the budget");
-            const int per_cu = s->tab ? std::max(1, std::min(s->occ == 2 ? 2 : 1, (int)((160 * 1024) / need))) : 0;
+            const int per_cu = s->tab ? std::max(1, std::min(s->occ == 3 ? 3 : 2, (int)((160 * 1024) / need))) : 0;
             s->max_groups = per_cu * m->cu_count;
         }
         // the fused forward pass when its weights fit in LDS;

@@ -595,15 +595,15 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
     }
 }
 
-// Registers unrestricted, or at most 256 (two waves per SIMD); see the run
-// kernel.
+// Registers unrestricted (two waves per SIMD), or at most 168 (three waves
+// per SIMD); see the run kernel.
 template <int C, bool TAB, bool REC>
 __global__ __launch_bounds__(256) void is_advance_kernel(IsParams P) {
     is_advance_body<C, TAB, REC>(P);
 }
 
 template <int C, bool TAB, bool REC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void is_advance_kernel_o2(IsParams P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void is_advance_kernel_o3(IsParams P) {
     is_advance_body<C, TAB, REC>(P);
 }
 

@@ -498,12 +498,12 @@ hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, 
     int grid = (row_groups + W - 1) / W;
     if (p.table_bytes && max_groups > 0 && grid > max_groups) grid = max_groups;
     IsAdvanceLaunch f = nullptr;
-    const bool o2 = occ == 2;
+    const bool o3 = occ == 3;
     switch (slots) {
-    case 1: f = o2 ? is_advance_c1_o2() : is_advance_c1_o0(); break;
-    case 2: f = o2 ? is_advance_c2_o2() : is_advance_c2_o0(); break;
-    case 4: f = o2 ? is_advance_c4_o2() : is_advance_c4_o0(); break;
-    case 8: f = o2 ? is_advance_c8_o2() : is_advance_c8_o0(); break;
+    case 1: f = o3 ? is_advance_c1_o3() : is_advance_c1_o0(); break;
+    case 2: f = o3 ? is_advance_c2_o3() : is_advance_c2_o0(); break;
+    case 4: f = o3 ? is_advance_c4_o3() : is_advance_c4_o0(); break;
+    case 8: f = o3 ? is_advance_c8_o3() : is_advance_c8_o0(); break;
     default: return hipErrorInvalidValue;
     }
     return f(p, grid, W, lds, s);

@@ -104,10 +104,10 @@ IsAdvanceLaunch is_advance_c1_o0();
 IsAdvanceLaunch is_advance_c2_o0();
 IsAdvanceLaunch is_advance_c4_o0();
 IsAdvanceLaunch is_advance_c8_o0();
-IsAdvanceLaunch is_advance_c1_o2();
-IsAdvanceLaunch is_advance_c2_o2();
-IsAdvanceLaunch is_advance_c4_o2();
-IsAdvanceLaunch is_advance_c8_o2();
+IsAdvanceLaunch is_advance_c1_o3();
+IsAdvanceLaunch is_advance_c2_o3();
+IsAdvanceLaunch is_advance_c4_o3();
+IsAdvanceLaunch is_advance_c8_o3();
 
 // occ: 0 registers unrestricted, 2 at most 256 (two waves per SIMD)
 hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, int max_groups, hipStream_t s);

@@ -18,10 +18,10 @@ hipError_t launch(const IsParams &p, int grid, int group_waves, size_t lds, hipS
     // builds: staged tables (sampling), global tables (sampling), global
     // tables with recording
     if (p.rec_pos && p.table_bytes) return hipErrorInvalidValue;
-#if SMC_OCC == 2
-    if (p.rec_pos) hipLaunchKernelGGL((is_advance_kernel_o2<SMC_C, false, true>), g, b, lds, s, p);
-    else if (p.table_bytes) hipLaunchKernelGGL((is_advance_kernel_o2<SMC_C, true, false>), g, b, lds, s, p);
-    else hipLaunchKernelGGL((is_advance_kernel_o2<SMC_C, false, false>), g, b, lds, s, p);
+#if SMC_OCC == 3
+    if (p.rec_pos) hipLaunchKernelGGL((is_advance_kernel_o3<SMC_C, false, true>), g, b, lds, s, p);
+    else if (p.table_bytes) hipLaunchKernelGGL((is_advance_kernel_o3<SMC_C, true, false>), g, b, lds, s, p);
+    else hipLaunchKernelGGL((is_advance_kernel_o3<SMC_C, false, false>), g, b, lds, s, p);
 #else
     if (p.rec_pos) hipLaunchKernelGGL((is_advance_kernel<SMC_C, false, true>), g, b, lds, s, p);
     else if (p.table_bytes) hipLaunchKernelGGL((is_advance_kernel<SMC_C, true, false>), g, b, lds, s, p);

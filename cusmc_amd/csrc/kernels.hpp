@@ -70,11 +70,11 @@ SimVariant sim_variant_c2_o0();
 SimVariant sim_variant_c4_o0();
 SimVariant sim_variant_c8_o0();
 SimVariant sim_variant_c16_o0();
-SimVariant sim_variant_c1_o2();
-SimVariant sim_variant_c2_o2();
-SimVariant sim_variant_c4_o2();
-SimVariant sim_variant_c8_o2();
-SimVariant sim_variant_c16_o2();
+SimVariant sim_variant_c1_o3();
+SimVariant sim_variant_c2_o3();
+SimVariant sim_variant_c4_o3();
+SimVariant sim_variant_c8_o3();
+SimVariant sim_variant_c16_o3();
 
 // occ: 0 registers unrestricted, 2 at most 256 (two waves per SIMD); tab:
 // tables staged in LDS (lp.table_bytes > 0); waves: total, a multiple of

@@ -81,6 +81,10 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
     unsigned long long g = 0;
     if (lane == 0) g = atomicAdd(lp.counter, 1ull);
     g = __shfl(g, 0);
+    // the same on every lane: make it a scalar, so the instance lookup and
+    // its tables and parameters live in scalar registers
+    g = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(g >> 32)) << 32) |
+        (unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)g);
     if (g >= lp.total_chunks) break;
     int lo = 0, hi = lp.n_inst - 1;
     while (lo < hi) {
@@ -433,17 +437,17 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
   }
 }
 
-// Two register policies of the run kernel: unrestricted (one wave per SIMD
-// once the kernel needs more than 256 registers), and at most 256 (two waves
-// per SIMD, spilling what does not fit).  Which is faster depends on the
-// packing; the autotuner measures both.
+// Two register policies of the run kernel: unrestricted (two waves per SIMD
+// for every slot count: 150-256 VGPRs), and at most 168 (three waves per
+// SIMD; up to four slots per lane fit, more spill).  Which is faster depends
+// on the packing; the autotuner measures both.
 template <int C, bool TRACE, bool TAB>
 __global__ __launch_bounds__(64 * SMC_MAX_GROUP_WAVES) void sim_kernel(LaunchParams lp) {
     sim_body<C, TRACE, TAB>(lp);
 }
 
 template <int C, bool TRACE, bool TAB>
-__global__ __launch_bounds__(64 * SMC_MAX_GROUP_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void sim_kernel_o2(
+__global__ __launch_bounds__(64 * SMC_MAX_GROUP_WAVES) __attribute__((amdgpu_waves_per_eu(3))) void sim_kernel_o3(
     LaunchParams lp) {
     sim_body<C, TRACE, TAB>(lp);
 }

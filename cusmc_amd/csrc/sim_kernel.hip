@@ -98,14 +98,14 @@ __global__ void prep_kernel(unsigned long long *counter, uint64_t *round, int ad
 namespace smc {
 
 static SimVariant variant(int slots, int occ) {
-    if (occ != 0 && occ != 2) return SimVariant{};
-    const bool o2 = occ == 2;
+    if (occ != 0 && occ != 3) return SimVariant{};
+    const bool o3 = occ == 3;
     switch (slots) {
-    case 1: return o2 ? sim_variant_c1_o2() : sim_variant_c1_o0();
-    case 2: return o2 ? sim_variant_c2_o2() : sim_variant_c2_o0();
-    case 4: return o2 ? sim_variant_c4_o2() : sim_variant_c4_o0();
-    case 8: return o2 ? sim_variant_c8_o2() : sim_variant_c8_o0();
-    case 16: return o2 ? sim_variant_c16_o2() : sim_variant_c16_o0();
+    case 1: return o3 ? sim_variant_c1_o3() : sim_variant_c1_o0();
+    case 2: return o3 ? sim_variant_c2_o3() : sim_variant_c2_o0();
+    case 4: return o3 ? sim_variant_c4_o3() : sim_variant_c4_o0();
+    case 8: return o3 ? sim_variant_c8_o3() : sim_variant_c8_o0();
+    case 16: return o3 ? sim_variant_c16_o3() : sim_variant_c16_o0();
     default: return SimVariant{};
     }
 }

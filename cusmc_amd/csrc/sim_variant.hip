@@ -1,6 +1,6 @@
 // sim_variant.hip — one build of the run kernel (sim_body.h): SMC_C
-// component slots per lane, register policy SMC_OCC (0 unrestricted, 2 at
-// most 256 registers: two waves per SIMD): with digests (tables in global
+// component slots per lane, register policy SMC_OCC (0 unrestricted, 3 at
+// most 168 registers: three waves per SIMD): with digests (tables in global
 // memory), and without, with the tables in global memory or staged in LDS.
 // The build compiles this file once per pair, in parallel.
 #include "sim_body.h"
@@ -14,8 +14,8 @@
 
 namespace {
 
-#if SMC_OCC == 2
-#define SMC_KERNEL sim_kernel_o2
+#if SMC_OCC == 3
+#define SMC_KERNEL sim_kernel_o3
 #else
 #define SMC_KERNEL sim_kernel
 #endif

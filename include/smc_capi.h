@@ -75,8 +75,9 @@ typedef struct smc_launch {
     int32_t lanes_per_run;     /* S: power of two <= 64                    */
     int32_t waves;             /* persistent waves in the grid              */
     int32_t waves_per_cu;      /* occupancy target used when waves == 0     */
-    int32_t occupancy;         /* 0: registers unrestricted; 2: at most 256
-                                  registers, two waves per SIMD            */
+    int32_t occupancy;         /* 0: registers unrestricted (two waves per
+                                  SIMD); 3: at most 168 registers, three
+                                  waves per SIMD                           */
     int32_t tables;            /* 0: model tables read from global memory;
                                   1: staged in LDS, one copy per workgroup
                                   of four waves (smc_run only; batches read

@@ -130,7 +130,7 @@ def test_lds_tables_bit_exact_vs_oracle(name, models_dir):
     sim = cs.GpuSimulator(net)
     try:
         for S in lane_opts:
-            for occ in (0, 2):
+            for occ in (0, 3):
                 print(f"[gpu parity, tables in LDS] {name} lanes_per_run={S} occupancy={occ} runs={runs}", flush=True)
                 res = sim.run(q, seed=seed, n_runs=runs, run_begin=3, lanes_per_run=S, occupancy=occ, tables=1,
                               per_run=keys)
