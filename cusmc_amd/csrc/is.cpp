@@ -138,7 +138,7 @@ struct smc_is {
     smc_model *m = nullptr;
     smc_query q{};
     smc_is_config cfg{};
-    int S = 1, C = 1, R = 64, stride = 1, occ = 0;
+    int S = 1, C = 1, R = 64, stride = 1;
     size_t lds = 0;
     uint32_t wave_lds = 0;
     bool tab = false;            // advance kernel stages the model tables in LDS
@@ -408,7 +408,7 @@ struct smc_is {
             unsigned int *cnt = counters.p + 3 + (k & 1);
             hip_check(hipMemsetAsync(cnt, 0, sizeof(unsigned int), s), "clear active count");
             P.active_count = cnt;
-            hip_check(smc::is_launch_advance(P, C, occ, lds_use, max_groups, s), "advance kernel launch");
+            hip_check(smc::is_launch_advance(P, C, lds_use, max_groups, s), "advance kernel launch");
             ++launches;
             if (!b.lockstep) break;
             hip_check(hipMemcpyAsync(h_counters + 2 + (k & 1), cnt, sizeof(unsigned int), hipMemcpyDeviceToHost, s),
@@ -557,7 +557,6 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         if (m->has_tuned && m->tuned.lanes_per_run > 0 && m->tuned.lanes_per_run <= s->S &&
             P / m->tuned.lanes_per_run <= 8) {
             s->S = m->tuned.lanes_per_run;
-            s->occ = m->tuned.occupancy == 3 ? 3 : 0;
         }
         // SMC_IS_LANES=<power of two> overrides the lanes per run (diagnosis:
         // the sampler's best packing may differ from the plain simulator's)
@@ -619,10 +618,14 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
         }
         s->Gpad = (int)rup((size_t)std::max(s->n_groups, 1), 64);
         s->G16 = (int)rup((size_t)std::max(s->n_groups, 1), 16);
-        // advance kernel LDS: per wave the variables of its 64/S rows and
-        // their tilt tables (theta and 2^theta, up to Gpad entries), after a
-        // shared part (tilt index and a shared 2^theta vector, Lpad each)
-        s->wave_lds = (uint32_t)rup((size_t)s->R * s->stride * sizeof(double) + (size_t)s->R * s->Gpad * 12, 16);
+        // advance kernel LDS: per wave the variables of its 64/S rows, their
+        // tilt tables (theta and 2^theta, up to Gpad entries) and the race
+        // tree's levels (2 log2(S) doubles per lane), after a shared part
+        // (tilt index and a shared 2^theta vector, Lpad each)
+        int levels = 0;
+        while ((1 << levels) < s->S) ++levels;
+        s->wave_lds = (uint32_t)rup((size_t)s->R * s->stride * sizeof(double) + (size_t)s->R * s->Gpad * 12 +
+                                        (size_t)2 * levels * 64 * sizeof(double), 16);
         if (s->wave_lds > 60000) throw std::runtime_error("limit: variables exceed the LDS budget");
         {
             const size_t shared = (size_t)s->Lpad * 12;
@@ -635,7 +638,8 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
             s->tab = want && m->table_bytes > 0 && need <= 160 * 1024;
             s->lds = s->tab ? need : shared + s->wave_lds;
             if (s->lds > 160 * 1024) throw std::runtime_error("limit: the advance kernel's LDS exceeds the budget");
-            const int per_cu = s->tab ? std::max(1, std::min(s->occ == 3 ? 3 : 2, (int)((160 * 1024) / need))) : 0;
+            // two waves per SIMD: two workgroups of four waves per CU
+            const int per_cu = s->tab ? std::max(1, std::min(2, (int)((160 * 1024) / need))) : 0;
             s->max_groups = per_cu * m->cu_count;
         }
         // the fused forward pass when its weights fit in LDS;

@@ -95,6 +95,7 @@ struct TiltLds {
     const double *ex;     // [theta_len]: 2^theta, shared theta only
     double *row_ex;       // the wave's rows: [64/S][theta_cap] 2^theta
     float *row_th;        // [64/S][theta_cap] theta
+    double *lvl;          // the wave's race-tree levels: [2 log2(S)][64] (own, sibling per level)
     double *rec;          // recording build: [C][64] rates, [C][64] int32 locations, [C][64] uint32 wins
 };
 
@@ -312,19 +313,26 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                             b &= b - 1;
                         }
                     }
-                    // tilted race: the run kernel's tree over the tilted rates
-                    double tr[2 * C];
+                    // tilted race: the run kernel's tree over the tilted rates;
+                    // the butterfly's partial sums wait in LDS for the descent
+                    double *lvl = tl.lvl + lane;
+                    double s;
+                    {
+                        double tr[2 * C];
 #pragma unroll
-                    for (int j = 0; j < C; ++j) tr[C + j] = leaft[j];
+                        for (int j = 0; j < C; ++j) tr[C + j] = leaft[j];
 #pragma unroll
-                    for (int i = C - 1; i >= 1; --i) tr[i] = tr[2 * i] + tr[2 * i + 1];
-                    double s = (C == 1) ? leaft[0] : tr[1];
-                    double own[6], sib[6];
+                        for (int i = C - 1; i >= 1; --i) tr[i] = tr[2 * i] + tr[2 * i + 1];
+                        s = (C == 1) ? leaft[0] : tr[1];
+                    }
 #pragma unroll
                     for (int k = 0; k < 6; ++k) {
-                        own[k] = s;
-                        sib[k] = 0.0;
-                        if ((1 << k) < S) { sib[k] = seg_xchg(s, k); s = s + sib[k]; }
+                        if ((1 << k) < S) {
+                            const double sib = seg_xchg(s, k);
+                            lvl[64 * (2 * k)] = s;
+                            lvl[64 * (2 * k + 1)] = sib;
+                            s = s + sib;
+                        }
                     }
                     const double Rt = s;
                     const double R = theta ? tree_total<C>(leaf, S) : Rt;
@@ -339,11 +347,18 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                         for (int k = 5; k >= 0; --k) {
                             if ((1 << k) < S) {
                                 const bool in_left = ((sl >> k) & 1) == 0;
-                                const double left = in_left ? own[k] : sib[k];
-                                const double right = in_left ? sib[k] : own[k];
+                                const double own = lvl[64 * (2 * k)], sib = lvl[64 * (2 * k + 1)];
+                                const double left = in_left ? own : sib;
+                                const double right = in_left ? sib : own;
                                 if (!(v < left || right == 0.0)) { v = v - left; b += 1 << k; }
                             }
                         }
+                        // the lane's tree again, from the same leaves (same sums)
+                        double tr[2 * C];
+#pragma unroll
+                        for (int j = 0; j < C; ++j) tr[C + j] = leaft[j];
+#pragma unroll
+                        for (int i = C - 1; i >= 1; --i) tr[i] = tr[2 * i] + tr[2 * i + 1];
                         int node = 1;
 #pragma unroll
                         for (int lv = 0; (1 << lv) < C; ++lv) {
@@ -562,8 +577,9 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
     extern __shared__ __attribute__((aligned(16))) double lds[];
     const int wv = threadIdx.x >> 6, W = blockDim.x >> 6;
     // LDS: [tables: TAB only][tilt index: lpad int32][2^theta: lpad doubles]
-    //      then per wave [variables][rows' 2^theta][rows' theta][recording
-    //      build: 16 C bytes per lane]
+    //      then per wave [variables][rows' 2^theta][rows' theta][race-tree
+    //      levels: 2 log2(S) doubles per lane][recording build: 16 C bytes
+    //      per lane]
     double *shared = lds + (TAB ? P.table_bytes / 8 : 0);
     int32_t *tix = (int32_t *)shared;
     double *ex = shared + P.lpad / 2;
@@ -574,7 +590,10 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
     tl.ex = ex;
     tl.row_ex = wlds + (size_t)R * P.var_stride;
     tl.row_th = (float *)(tl.row_ex + (size_t)R * P.theta_cap);
-    tl.rec = (double *)(tl.row_th + (size_t)R * P.theta_cap);
+    tl.lvl = (double *)(tl.row_th + (size_t)R * P.theta_cap);
+    int levels = 0;
+    while ((1 << levels) < P.lanes) ++levels;
+    tl.rec = tl.lvl + (size_t)2 * levels * 64;
     for (int i = threadIdx.x; i < P.lpad; i += blockDim.x) tix[i] = P.tilt_index[i];
     if (P.theta && P.theta_stride == 0)
         for (int i = threadIdx.x; i < P.theta_len; i += blockDim.x) ex[i] = smc_exp2((double)P.theta[i]);
@@ -595,15 +614,9 @@ __device__ __forceinline__ void is_advance_body(const IsParams &P) {
     }
 }
 
-// Registers unrestricted (two waves per SIMD), or at most 168 (three waves
-// per SIMD); see the run kernel.
+// Registers unrestricted: 190-250 VGPRs, two waves per SIMD.
 template <int C, bool TAB, bool REC>
 __global__ __launch_bounds__(256) void is_advance_kernel(IsParams P) {
-    is_advance_body<C, TAB, REC>(P);
-}
-
-template <int C, bool TAB, bool REC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void is_advance_kernel_o3(IsParams P) {
     is_advance_body<C, TAB, REC>(P);
 }
 

@@ -487,7 +487,7 @@ __global__ __launch_bounds__(64 * kMlpWaves) void mlp_forward_kernel(MlpFwdArgs 
 
 namespace smc {
 
-hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, int max_groups, hipStream_t s) {
+hipError_t is_launch_advance(const IsParams &p, int slots, size_t lds, int max_groups, hipStream_t s) {
     const int R = 64 / p.lanes;
     const int row_groups = (p.n_rows + R - 1) / R;
     if (row_groups <= 0) return hipSuccess;
@@ -498,12 +498,11 @@ hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, 
     int grid = (row_groups + W - 1) / W;
     if (p.table_bytes && max_groups > 0 && grid > max_groups) grid = max_groups;
     IsAdvanceLaunch f = nullptr;
-    const bool o3 = occ == 3;
     switch (slots) {
-    case 1: f = o3 ? is_advance_c1_o3() : is_advance_c1_o0(); break;
-    case 2: f = o3 ? is_advance_c2_o3() : is_advance_c2_o0(); break;
-    case 4: f = o3 ? is_advance_c4_o3() : is_advance_c4_o0(); break;
-    case 8: f = o3 ? is_advance_c8_o3() : is_advance_c8_o0(); break;
+    case 1: f = is_advance_c1_o0(); break;
+    case 2: f = is_advance_c2_o0(); break;
+    case 4: f = is_advance_c4_o0(); break;
+    case 8: f = is_advance_c8_o0(); break;
     default: return hipErrorInvalidValue;
     }
     return f(p, grid, W, lds, s);

@@ -104,13 +104,8 @@ IsAdvanceLaunch is_advance_c1_o0();
 IsAdvanceLaunch is_advance_c2_o0();
 IsAdvanceLaunch is_advance_c4_o0();
 IsAdvanceLaunch is_advance_c8_o0();
-IsAdvanceLaunch is_advance_c1_o3();
-IsAdvanceLaunch is_advance_c2_o3();
-IsAdvanceLaunch is_advance_c4_o3();
-IsAdvanceLaunch is_advance_c8_o3();
 
-// occ: 0 registers unrestricted, 2 at most 256 (two waves per SIMD)
-hipError_t is_launch_advance(const IsParams &p, int slots, int occ, size_t lds, int max_groups, hipStream_t s);
+hipError_t is_launch_advance(const IsParams &p, int slots, size_t lds, int max_groups, hipStream_t s);
 // Reset rows [0, rows) to the initial state of runs runs[i] (or run_begin + i);
 // rows >= n_valid are inactive.  feat0 is the feature row of the initial state.
 hipError_t is_init_rows(const IsParams &p, const uint64_t *runs, uint64_t run_begin, int n_valid,

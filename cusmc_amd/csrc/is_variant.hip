@@ -1,11 +1,11 @@
 // is_variant.hip — one build of the tilted advance kernel (is_advance.h):
-// SMC_C component slots per lane, register policy SMC_OCC (0 unrestricted,
-// 2 at most 256 registers), with the tables in global memory or staged in
-// LDS.  The build compiles this file once per pair.
+// SMC_C component slots per lane (registers unrestricted, SMC_OCC 0), with
+// the tables in global memory or staged in LDS.  The build compiles this
+// file once per slot count.
 #include "is_advance.h"
 
-#if !defined(SMC_C) || !defined(SMC_OCC)
-#error "SMC_C and SMC_OCC must be defined"
+#if !defined(SMC_C) || !defined(SMC_OCC) || SMC_OCC != 0
+#error "SMC_C must be defined and SMC_OCC be 0"
 #endif
 
 #define SMC_JOIN2(a, b, c, d) a##b##c##d
@@ -18,15 +18,9 @@ hipError_t launch(const IsParams &p, int grid, int group_waves, size_t lds, hipS
     // builds: staged tables (sampling), global tables (sampling), global
     // tables with recording
     if (p.rec_pos && p.table_bytes) return hipErrorInvalidValue;
-#if SMC_OCC == 3
-    if (p.rec_pos) hipLaunchKernelGGL((is_advance_kernel_o3<SMC_C, false, true>), g, b, lds, s, p);
-    else if (p.table_bytes) hipLaunchKernelGGL((is_advance_kernel_o3<SMC_C, true, false>), g, b, lds, s, p);
-    else hipLaunchKernelGGL((is_advance_kernel_o3<SMC_C, false, false>), g, b, lds, s, p);
-#else
     if (p.rec_pos) hipLaunchKernelGGL((is_advance_kernel<SMC_C, false, true>), g, b, lds, s, p);
     else if (p.table_bytes) hipLaunchKernelGGL((is_advance_kernel<SMC_C, true, false>), g, b, lds, s, p);
     else hipLaunchKernelGGL((is_advance_kernel<SMC_C, false, false>), g, b, lds, s, p);
-#endif
     return hipGetLastError();
 }
 
