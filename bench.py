@@ -43,7 +43,7 @@ sys.path.insert(0, str(ROOT))
 LANE_PEAK = 256 * 64 * 2.4e9        # VALU lane-operations per second
 WAVE_PEAK = 256 * 2.4e9             # wave64 VALU instructions per second
 # algorithmic lane-ops per race step (oracle cost model; tests pin them)
-COST_PER_STEP = {"fischer32": 599.9, "sir350": 2215.0, "race8": 639.9}
+COST_PER_STEP = {"fischer32": 598.2, "sir350": 2214.6, "race8": 639.4}
 SEED = 20261202
 # Counter figures of the run kernel from rocprofv3 runs of this file (the
 # commands and raw values are in profiles/), for the launch shape (lanes per

@@ -225,7 +225,10 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     if (m.guarded) blk = blocked_slots<C>(loc, vars, m, sl, n);
                     if (!full) {
                         changed = false;
-                        if (absorbed<C>(loc, vars, m, sl, n, seg_mask, blk)) { end = true; break; }
+                        // with frozen racers the race itself ends a run that nothing
+                        // can leave (no event: blocked racers are out, the rest race at
+                        // rate 0), so the scan is needed only without them
+                        if (!m.freeze && absorbed<C>(loc, vars, m, sl, n, seg_mask, blk)) { end = true; break; }
                     }
                 }
 
@@ -383,7 +386,8 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     }
                     bool u_won = false;
                     if (gu < dexp || (gu == dexp && wu < wexp)) { d = gu; w = wu; u_won = true; }
-                    else { d = dexp; w = wexp; exp_won = true; }
+                    else if (dexp < INF) { d = dexp; w = wexp; exp_won = true; }
+                    else { d = INF; w = -1; }  // no racer: no event
                     // the step that crosses the bound: the left-out null
                     // racers' first event after it (base rates, no tilt)
                     if (!full && d < INF && !(t + d < P.bound) && (__ballot(deadmask != 0) & seg_mask)) {
@@ -439,7 +443,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                         }
                     }
                 }
-                if (!(d < INF)) { t = INF; end = true; break; }
+                if (!(d < INF)) { end = true; break; }  // no event: the run ends at its clock
                 if (sl == 0) vars[-1] = t + d;  // clocks advance
                 wave_sync();
                 if (w < 0) break;  // a null event of the left-out racers

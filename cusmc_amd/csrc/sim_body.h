@@ -185,7 +185,10 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                         if (m.guarded) blk = blocked_slots<C>(loc, vars, m, sl, n);
                         if (!full) {
                             changed = false;
-                            if (absorbed<C>(loc, vars, m, sl, n, seg_mask, blk)) { end = true; break; }
+                            // with frozen racers the race itself ends a run that nothing
+                            // can leave (no event: blocked racers are out, the rest race at
+                            // rate 0), so the scan is needed only without them
+                            if (!m.freeze && absorbed<C>(loc, vars, m, sl, n, seg_mask, blk)) { end = true; break; }
                         }
                     }
 
@@ -330,7 +333,8 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                             wu = (Lu - seg_base) * C + __shfl(unif_j, Lu);
                         }
                         if (gu < dexp || (gu == dexp && wu < wexp)) { d = gu; w = wu; }
-                        else { d = dexp; w = wexp; }
+                        else if (dexp < INF) { d = dexp; w = wexp; }
+                        else { d = INF; w = -1; }  // no racer: no event
                         // the step that crosses the bound: the left-out null
                         // racers' first event after the bound may come first
                         if (!full && d < INF && !(t + d < p.bound) && (__ballot(deadmask != 0) & seg_mask)) {
@@ -342,7 +346,7 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                             }
                         }
                     }
-                    if (!(d < INF)) { t = INF; end = true; break; }
+                    if (!(d < INF)) { end = true; break; }  // no event: the run ends at its clock
                     // clocks advance (find_minimum_delay :632-651): one store of
                     // the run's clock time
                     if (sl == 0) vars[-1] = t + d;

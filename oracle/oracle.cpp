@@ -369,14 +369,17 @@ public:
             // exit and no left-out null racers.
             const bool goal_pending = reach && goal;
             // absorption (canonical sampling; cuSMC keeps racing null
-            // steps to the bound, with the same outcome)
+            // steps to the bound, with the same outcome).  With frozen
+            // racers (freeze_) the race below finds such a state by itself:
+            // blocked components leave it, the rest race at rate 0, so it
+            // has no event and ends the run.
             if (mode_ == ORACLE_CANONICAL && changed) {
                 // blocked components: unchanged until the next transition
                 for (int c = 0; c < N; ++c)
                     blocked_[c] = n_.locations[loc_[c]].type <= SMC_LOC_GOAL && blocked(c);
                 if (!goal_pending) {
                     changed = false;
-                    if (absorbed()) break;
+                    if (!freeze_ && absorbed()) break;
                 }
             }
 
@@ -386,7 +389,7 @@ public:
             if (mode_ == ORACLE_LITERAL) race_literal(run, step, &d, &w);
             else race_canonical(run, step, t, goal_pending, &d, &w, &won);
             if (record && won) ++wins[w];
-            if (!(d < HUGE_VAL)) { t = HUGE_VAL; break; }
+            if (!(d < HUGE_VAL)) break;  // no event: the run ends at its clock
 
             // clocks advance: the clock time moves, every clock's value is
             // the clock time minus its stamp
@@ -705,7 +708,8 @@ private:
         if (wu < 0) wu = 0x7FFFFFFF;
         bool won = false, uwon = false;
         if (du < dexp || (du == dexp && wu < wexp)) { *d_out = du; *w_out = wu; uwon = true; }
-        else { *d_out = dexp; *w_out = wexp; won = true; }
+        else if (dexp < HUGE_VAL) { *d_out = dexp; *w_out = wexp; won = true; }
+        else { *d_out = HUGE_VAL; *w_out = -1; }  // no racer: no event
         // the step that crosses the bound: the left-out null racers' first
         // event after the bound (base rates) may come first; then the step
         // is a null step (w = -1)
