@@ -344,14 +344,32 @@ ModelImage compile_model(const smc_network &n) {
         img.locs[l].flags = (uint8_t)((tf ? SMC_LF_TIME_FREE : 0) | (kind == 1 ? SMC_LF_DEAD : 0) |
                                       (kind == 2 ? SMC_LF_GUARDED : 0));
         if (kind == 2 && img.locs[l].klass != SMC_K_ZERO) img.guarded = true;
+        if (kind == 2) {
+            // the blocking test: clock-free conjuncts, edge after edge
+            img.locs[l].block_first = (uint32_t)img.guards.size();
+            for (int k = 0; k < L.edge_count; ++k) {
+                const smc_edge &E = n.edges[L.edge_first + k];
+                if (E.channel < 0) continue;
+                const DevEdge &DE = img.edges[L.edge_first + k];
+                if (DE.never) continue;
+                const size_t first = img.guards.size();
+                for (int g = 0; g < DE.guard_n; ++g) {
+                    const DevGuard G = img.guards[DE.guard_first + g];
+                    if (G.flags & SMC_GF_CLOCK_FREE) img.guards.push_back(G);
+                }
+                if (img.guards.size() == first) throw std::runtime_error("internal: guarded edge without a clock-free conjunct");
+                img.guards.back().flags |= SMC_GF_END_EDGE;
+            }
+            img.guards.back().flags |= SMC_GF_END_LOC;
+        }
     }
     img.freeze = !clock_rate;
     for (int l = 0; l < n.n_locations; ++l) {
         const smc_location &L = n.locations[l];
-        uint32_t bits = 0;
-        for (int k = 0; k < L.edge_count && k < 32; ++k)
-            if (fast_edge[L.edge_first + k]) bits |= 1u << k;
-        img.locs[l].recv_fast = bits;
+        bool fast = true;
+        for (int k = 0; k < L.edge_count; ++k)
+            if (n.edges[L.edge_first + k].channel < 0 && !fast_edge[L.edge_first + k]) fast = false;
+        if (fast) img.locs[l].flags |= SMC_LF_RECV_FAST;
     }
     // keep every table non-empty so device pointers are valid
     if (img.invs.empty()) img.invs.push_back(DevInv{});

@@ -250,6 +250,36 @@ __device__ __forceinline__ double location_rate(const DevLoc &L, const double *v
     return memo.value;
 }
 
+// Rates of the slots in `pend` whose rate is a bytecode expression (refs
+// in ref[]): each pass evaluates, on every lane at once, the expression of
+// the lane's first pending slot and hands the value to all of its pending
+// slots with the same expression.  Evaluating inside the per-slot loop would
+// run the interpreter once per slot index at which any lane of the wave
+// meets a new expression (up to C times per race); here it runs once per
+// distinct expression a lane holds.
+template <int C, class M>
+__device__ __forceinline__ void pending_rates(uint32_t pend, const uint32_t (&ref)[C], double (&leaf)[C],
+                                              const double *vars, const M &m) {
+    while (__ballot(pend != 0)) {
+        uint32_t r = 0xFFFFFFFFu;
+        double v = 0.0;
+        if (pend) {
+            const int j0 = __ffs(pend) - 1;
+            r = ref[0];
+#pragma unroll
+            for (int k = 1; k < C; ++k) r = (j0 == k) ? ref[k] : r;
+            v = eval_ref(r, vars, m);
+            v = v > 0.0 ? v : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < C; ++j)
+            if (((pend >> j) & 1u) && ref[j] == r) {
+                leaf[j] = v;
+                pend &= ~(1u << j);
+            }
+    }
+}
+
 // ------------------------------------------------------------ edges
 
 __device__ __forceinline__ bool guard_holds(int op, double x, double r) {
@@ -399,17 +429,18 @@ __device__ __forceinline__ uint32_t blocked_slots(const int (&loc)[C], const dou
         const DevLoc L = load_rec(m.locs + loc[j]);
         bool blk = false;
         if ((L.flags & SMC_LF_GUARDED) && L.klass != SMC_K_ZERO) {
+            // every edge has a false clock-free conjunct: one pass over the
+            // location's blocking records
             blk = true;
-            for (int i = 0; i < L.edge_n && blk; ++i) {
-                const DevEdge E = load_rec(m.edges + L.edge_first + i);
-                if (E.never || E.channel < 0) continue;
-                bool e_blk = false;
-                for (int g = 0; g < E.guard_n && !e_blk; ++g) {
-                    const DevGuard G = load_rec(m.guards + E.guard_first + g);
-                    if (G.flags & SMC_GF_CLOCK_FREE)
-                        e_blk = !guard_holds(G.op, vars[G.var], eval_ref(G.rhs, vars, m));
+            bool e_blk = false;
+            for (uint32_t k = L.block_first;; ++k) {
+                const DevGuard G = load_rec(m.guards + k);
+                if (!e_blk) e_blk = !guard_holds(G.op, vars[G.var], eval_ref(G.rhs, vars, m));
+                if (G.flags & SMC_GF_END_EDGE) {
+                    if (!e_blk) { blk = false; break; }
+                    e_blk = false;
                 }
-                blk = e_blk;
+                if (G.flags & SMC_GF_END_LOC) break;
             }
         }
         if (blk) b |= 1u << j;
@@ -452,7 +483,7 @@ __device__ __forceinline__ void slot_set(T (&a)[C], int j, T v);
 
 // Receivers of one broadcast fired in parallel (SyncInSerial
 // simulation.cu:378-404 fires them one by one in component order).  Valid
-// when every enabled receive edge of the segment commutes (DevLoc.recv_fast):
+// when every receive edge of the segment's receivers commutes (SMC_LF_RECV_FAST):
 // constant weights make each receiver's pick independent of the others'
 // updates, and integer additions and private constants give the same
 // variables in any order.  The query variable's running max/min needs the
@@ -470,7 +501,7 @@ __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int
     bool slow = false;
 #pragma unroll
     for (int j = 0; j < C; ++j)
-        if (mk[j] && (mk[j] & ~(uint64_t)m.locs[loc[j]].recv_fast)) slow = true;
+        if (mk[j] && !(m.locs[loc[j]].flags & SMC_LF_RECV_FAST)) slow = true;
     if (__ballot(slow) & seg_mask) return false;
 
     int ej[C];

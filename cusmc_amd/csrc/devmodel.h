@@ -60,8 +60,9 @@ enum {
                             // change only with transitions
     SMC_LF_DEAD = 2,        // no spontaneous edge can ever fire (each has a `!=` conjunct, or
                             // there is none): always blocked
-    SMC_LF_GUARDED = 4      // blocked or not by the variables: every spontaneous edge without
-                            // `!=` has a conjunct that reads no clock
+    SMC_LF_GUARDED = 4,     // blocked or not by the variables: every spontaneous edge without
+                            // `!=` has a conjunct that reads no clock (DevLoc.block_first)
+    SMC_LF_RECV_FAST = 8    // every receive edge commutes (see DevUpd)
 };
 
 struct DevInsn {
@@ -81,7 +82,7 @@ struct DevLoc {             // 32 bytes, one pair of 16-byte loads
     uint16_t edge_n;
     uint8_t rate_const;
     uint8_t flags;          // SMC_LF_*
-    uint32_t recv_fast;     // bit i: edge i is a commuting receive edge (see DevUpd)
+    uint32_t block_first;   // SMC_LF_GUARDED: the blocking test, guard records from here on
 };
 
 struct DevInv {             // x <= bound  (strict: x < bound), 16 bytes
@@ -109,8 +110,15 @@ struct DevEdge {            // 32 bytes
 // DevGuard.flags
 enum {
     SMC_GF_CLOCK_FREE = 1,  // neither var nor rhs reads a clock: fixed between transitions
-    SMC_GF_CLOCK = 2        // var is a clock
+    SMC_GF_CLOCK = 2,       // var is a clock
+    SMC_GF_END_EDGE = 4,    // blocking test: last conjunct of its edge
+    SMC_GF_END_LOC = 8      // blocking test: last record of the location
 };
+
+// A guard-blocked location's blocking test is a run of guard records (from
+// DevLoc.block_first): the clock-free conjuncts of each spontaneous edge
+// without `!=`, edge after edge.  The location is blocked when every edge
+// has a false conjunct among them.
 
 struct DevGuard {           // variables[var] op rhs
     uint16_t var;
