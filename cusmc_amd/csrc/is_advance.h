@@ -233,12 +233,14 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                 }
 
                 double leaf[C], leaft[C], Mb[C];
+                uint32_t rref[C];  // rate expression of slots in rpend
                 RateMemo memo;
                 int zero_j = C;
-                uint32_t bmask = 0, deadmask = 0;
+                uint32_t bmask = 0, deadmask = 0, rpend = 0;
 #pragma unroll
                 for (int j = 0; j < C; ++j) {
                     leaf[j] = 0.0; leaft[j] = 0.0; Mb[j] = 0.0;
+                    rref[j] = 0xFFFFFFFFu;
                     const int c = sl * C + j;
                     if (c < n) {
                         const DevLoc L = load_rec(m.locs + loc[j]);
@@ -259,14 +261,27 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                                 else { Mb[j] = M; bmask |= 1u << j; }
                             } else if (m.freeze && ((L.flags & SMC_LF_DEAD) || ((blk >> j) & 1u))) {
                                 deadmask |= 1u << j;
+                            } else if (L.rate_const) {
+                                leaf[j] = L.rate_value > 0.0 ? L.rate_value : 0.0;
                             } else {
-                                const double r = location_rate(L, vars, m, memo);
-                                leaf[j] = r > 0.0 ? r : 0.0;
-                                leaft[j] = leaf[j];
-                                if (theta && leaf[j] > 0.0) {
-                                    const int ti = tl.tix[loc[j]];
-                                    if (ti >= 0) leaft[j] = leaf[j] * tex[ti];
-                                }
+                                rref[j] = L.rate;
+                                rpend |= 1u << j;
+                            }
+                        }
+                    }
+                }
+                {
+                    // racing exponential slots: rates (expressions evaluated once per
+                    // distinct expression), then their tilts
+                    uint32_t racing = rpend;
+                    pending_rates<C>(rpend, rref, leaf, vars, m);
+#pragma unroll
+                    for (int j = 0; j < C; ++j) {
+                        if (leaf[j] > 0.0 || ((racing >> j) & 1u)) {
+                            leaft[j] = leaf[j];
+                            if (theta && leaf[j] > 0.0) {
+                                const int ti = tl.tix[loc[j]];
+                                if (ti >= 0) leaft[j] = leaf[j] * tex[ti];
                             }
                         }
                     }

@@ -193,13 +193,15 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                     }
 
                     double leaf[C], Mb[C];
+                    uint32_t rref[C];     // rate expression of slots in rpend
                     int zero_j = C;
-                    uint32_t bmask = 0, deadmask = 0;
+                    uint32_t bmask = 0, deadmask = 0, rpend = 0;
                     RateMemo memo;
 #pragma unroll
                     for (int j = 0; j < C; ++j) {
                         leaf[j] = 0.0;
                         Mb[j] = 0.0;
+                        rref[j] = 0xFFFFFFFFu;
                         const int c = sl * C + j;
                         if (c < n) {
                             const DevLoc L = load_rec(m.locs + loc[j]);
@@ -220,13 +222,16 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                                     else { Mb[j] = M; bmask |= 1u << j; }
                                 } else if (m.freeze && ((L.flags & SMC_LF_DEAD) || ((blk >> j) & 1u))) {
                                     deadmask |= 1u << j;
+                                } else if (L.rate_const) {
+                                    leaf[j] = L.rate_value > 0.0 ? L.rate_value : 0.0;
                                 } else {
-                                    const double r = location_rate(L, vars, m, memo);
-                                    leaf[j] = r > 0.0 ? r : 0.0;
+                                    rref[j] = L.rate;
+                                    rpend |= 1u << j;
                                 }
                             }
                         }
                     }
+                    pending_rates<C>(rpend, rref, leaf, vars, m);
                     // A frozen component (blocked, exponential) only makes
                     // null steps until the next transition.  Leaving it out of
                     // the race keeps the law of everything else: exponential
