@@ -418,6 +418,22 @@ __device__ __forceinline__ int fire_edge(int e, double *vars, const M &m, LaneQu
 // kernels recompute the mask after every change of state, on models with
 // such locations (DevModel.guarded).  Locations flagged SMC_LF_DEAD are
 // always blocked and are tested by their flag.
+// The blocking test of a location flagged SMC_LF_GUARDED: every edge has a
+// false clock-free conjunct (one pass over its blocking records).
+template <class M>
+__device__ __forceinline__ bool location_blocked(const DevLoc &L, const double *vars, const M &m) {
+    bool e_blk = false;
+    for (uint32_t k = L.block_first;; ++k) {
+        const DevGuard G = load_rec(m.guards + k);
+        if (!e_blk) e_blk = !guard_holds(G.op, vars[G.var], eval_ref(G.rhs, vars, m));
+        if (G.flags & SMC_GF_END_EDGE) {
+            if (!e_blk) return false;
+            e_blk = false;
+        }
+        if (G.flags & SMC_GF_END_LOC) return true;
+    }
+}
+
 template <int C, class M>
 __device__ __forceinline__ uint32_t blocked_slots(const int (&loc)[C], const double *vars, const M &m, int sl,
                                                   int n) {
@@ -427,23 +443,7 @@ __device__ __forceinline__ uint32_t blocked_slots(const int (&loc)[C], const dou
         const int c = sl * C + j;
         if (c >= n) continue;
         const DevLoc L = load_rec(m.locs + loc[j]);
-        bool blk = false;
-        if ((L.flags & SMC_LF_GUARDED) && L.klass != SMC_K_ZERO) {
-            // every edge has a false clock-free conjunct: one pass over the
-            // location's blocking records
-            blk = true;
-            bool e_blk = false;
-            for (uint32_t k = L.block_first;; ++k) {
-                const DevGuard G = load_rec(m.guards + k);
-                if (!e_blk) e_blk = !guard_holds(G.op, vars[G.var], eval_ref(G.rhs, vars, m));
-                if (G.flags & SMC_GF_END_EDGE) {
-                    if (!e_blk) { blk = false; break; }
-                    e_blk = false;
-                }
-                if (G.flags & SMC_GF_END_LOC) break;
-            }
-        }
-        if (blk) b |= 1u << j;
+        if ((L.flags & SMC_LF_GUARDED) && L.klass != SMC_K_ZERO && location_blocked(L, vars, m)) b |= 1u << j;
     }
     return b;
 }

@@ -221,8 +221,11 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                 // a goal entered by receivers: this step is raced in full
                 // and decides the hit (as in the run kernel)
                 const bool full = P.reach && goal;
+                // blocked bits: after a change of state, in the slot loop
+                // below on models with frozen racers, here otherwise
+                const bool reblk = changed && m.guarded && m.freeze;
                 if (changed) {
-                    if (m.guarded) blk = blocked_slots<C>(loc, vars, m, sl, n);
+                    if (m.guarded && !m.freeze) blk = blocked_slots<C>(loc, vars, m, sl, n);
                     if (!full) {
                         changed = false;
                         // with frozen racers the race itself ends a run that nothing
@@ -244,6 +247,10 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     const int c = sl * C + j;
                     if (c < n) {
                         const DevLoc L = load_rec(m.locs + loc[j]);
+                        if (reblk && (L.flags & SMC_LF_GUARDED) && L.klass != SMC_K_ZERO) {
+                            const uint32_t bit = 1u << j;
+                            blk = location_blocked(L, vars, m) ? (blk | bit) : (blk & ~bit);
+                        }
                         if (L.klass == SMC_K_ZERO) {
                             if (zero_j == C) zero_j = j;
                         } else {

@@ -181,8 +181,12 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                     // t < bound (simulation.cu:409-415, 835-849): no
                     // absorption exit and no left-out null racers then
                     const bool full = p.reach && goal;
+                    // blocked bits: after a change of state, in the slot loop
+                    // below on models with frozen racers (it loads each slot's
+                    // location anyway), here for the absorption scan otherwise
+                    const bool reblk = changed && m.guarded && m.freeze;
                     if (changed) {
-                        if (m.guarded) blk = blocked_slots<C>(loc, vars, m, sl, n);
+                        if (m.guarded && !m.freeze) blk = blocked_slots<C>(loc, vars, m, sl, n);
                         if (!full) {
                             changed = false;
                             // with frozen racers the race itself ends a run that nothing
@@ -205,6 +209,10 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                         const int c = sl * C + j;
                         if (c < n) {
                             const DevLoc L = load_rec(m.locs + loc[j]);
+                            if (reblk && (L.flags & SMC_LF_GUARDED) && L.klass != SMC_K_ZERO) {
+                                const uint32_t bit = 1u << j;
+                                blk = location_blocked(L, vars, m) ? (blk | bit) : (blk & ~bit);
+                            }
                             if (L.klass == SMC_K_ZERO) {
                                 if (zero_j == C) zero_j = j;
                             } else {
