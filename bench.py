@@ -52,9 +52,9 @@ SEED = 20261202
 # what they read for a 1 GiB device copy).  valu: SQ_INSTS_VALU (wave64 VALU
 # instructions) per race step.
 PROFILED = {
-    "fischer32": {"shape": (4, 3, 1), "traffic_per_run": 25.3, "valu_per_step": 166.6,
+    "fischer32": {"shape": (4, 3, 1), "traffic_per_run": 23.3, "valu_per_step": 118.5,
                   "source": "profiles/sim_kernel_fischer32.md"},
-    "sir350": {"shape": (64, 3, 0), "traffic_per_run": 70.8, "valu_per_step": 2238.6,
+    "sir350": {"shape": (64, 3, 0), "traffic_per_run": 55.9, "valu_per_step": 1639.2,
                "source": "profiles/sim_kernel_sir350.md"},
 }
 
