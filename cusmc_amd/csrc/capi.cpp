@@ -249,6 +249,7 @@ int smc_model_create(const smc_network *net, int device, smc_model **out) {
             {im.comp_init_loc.data(), im.comp_init_loc.size() * sizeof(int32_t), 0},
             {im.var_clock.data(), im.var_clock.size() * sizeof(uint8_t), 0},
             {im.comp_first_loc.data(), im.comp_first_loc.size() * sizeof(int32_t), 0},
+            {im.loc_edge0.data(), im.loc_edge0.size() * sizeof(int32_t), 0},
         };
         size_t total = 0;
         for (auto &p : parts) { p.off = total; total = align_up(total + p.bytes, 256); }
@@ -270,6 +271,7 @@ int smc_model_create(const smc_network *net, int device, smc_model **out) {
         d.comp_init_loc = (decltype(d.comp_init_loc))at(8);
         d.var_clock = (decltype(d.var_clock))at(9);
         d.comp_first_loc = (decltype(d.comp_first_loc))at(10);
+        d.loc_edge0 = (decltype(d.loc_edge0))at(11);
         d.n_comp = im.n_comp;
         d.n_vars = im.n_vars;
         int nclk = 0;

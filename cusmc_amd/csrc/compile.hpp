@@ -21,6 +21,7 @@ struct ModelImage {
     std::vector<int32_t> comp_first_loc;
     std::vector<uint8_t> var_clock;     // per variable: 1 for a clock
     std::vector<int32_t> var_kind;
+    std::vector<int32_t> loc_edge0;     // per location: the network's index of its first edge
     int n_comp = 0, n_vars = 0, n_leaves = 1;
     bool freeze = true;   // no location's rate reads a clock
     bool guarded = false; // some location is blocked or not by the variables

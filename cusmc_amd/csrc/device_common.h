@@ -180,6 +180,7 @@ __device__ __forceinline__ DevModelT<SmcLdsTables> lds_model(const DevModel &mg,
     m.comp_init_loc = mg.comp_init_loc;
     m.var_clock = mg.var_clock;
     m.comp_first_loc = mg.comp_first_loc;
+    m.loc_edge0 = mg.loc_edge0;
     m.n_comp = mg.n_comp;
     m.n_vars = mg.n_vars;
     m.n_clocks = mg.n_clocks;
@@ -391,10 +392,11 @@ struct LaneQuery {
     double qmax, qmin;
 };
 
-// Fire edge `e`: updates in order, query variable tracking, new location
-// (take_transition simulation.cu:88-180).  Returns the destination.
+// Fire edge `e` of location `src`: updates in order, query variable
+// tracking, new location (take_transition simulation.cu:88-180).  Returns
+// the destination (edge records hold it relative to the source).
 template <class M>
-__device__ __forceinline__ int fire_edge(int e, double *vars, const M &m, LaneQuery &q,
+__device__ __forceinline__ int fire_edge(int e, int src, double *vars, const M &m, LaneQuery &q,
                                          bool &goal, int &chan) {
     const DevEdge E = load_rec(m.edges + e);
     for (int k = 0; k < E.upd_n; ++k) {
@@ -408,7 +410,7 @@ __device__ __forceinline__ int fire_edge(int e, double *vars, const M &m, LaneQu
     }
     if (E.dst_goal) goal = true;
     if (E.channel > 0) chan = E.channel;
-    return E.dst;
+    return src + E.dst;
 }
 
 // Slots (bit j) whose component is blocked by the variables: in a
@@ -504,7 +506,7 @@ __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int
         if (mk[j] && !(m.locs[loc[j]].flags & SMC_LF_RECV_FAST)) slow = true;
     if (__ballot(slow) & seg_mask) return false;
 
-    int ej[C];
+    int ej[C], eg[C];  // fired edge per slot: record index, and the network's index (digests)
     int fired = 0;
     bool any_fired = false;
     double qsum = 0.0, qhi = -DBL_MAX, qlo = DBL_MAX;  // prefix sums of this lane's query additions
@@ -517,6 +519,7 @@ __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int
 #pragma unroll
     for (int j = 0; j < C; ++j) {
         ej[j] = -1;
+        eg[j] = -1;
         const uint32_t c = (uint32_t)(sl * C + j);
         if (j % G == 0) {
             bool choice = false;
@@ -526,9 +529,11 @@ __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int
         }
         if (mk[j]) {
             const int first = (int)m.locs[loc[j]].edge_first;
-            const int e = first + ((mk[j] & (mk[j] - 1)) ? pick_weighted(mk[j], first, smc_receiver_u(rw, c), vars, m)
-                                                         : lowest_lane(mk[j]));
+            const int i = (mk[j] & (mk[j] - 1)) ? pick_weighted(mk[j], first, smc_receiver_u(rw, c), vars, m)
+                                                : lowest_lane(mk[j]);
+            const int e = first + i;
             ej[j] = e;
+            if (trace) eg[j] = m.loc_edge0[loc[j]] + i;
             ++fired;
             any_fired = true;
             const DevEdge E = load_rec(m.edges + e);
@@ -570,7 +575,7 @@ __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int
                 else vars[U.var] = U.clock ? vars[-1] - U.value : U.value;
             }
             if (E.dst_goal) goal = true;
-            loc[j] = E.dst;
+            loc[j] = loc[j] + E.dst;
         }
     }
     wave_sync();
@@ -581,7 +586,7 @@ __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int
             const int L = lowest_lane(lanes_with);
 #pragma unroll
             for (int j = 0; j < C; ++j) {
-                const int e = __shfl(ej[j], L);
+                const int e = __shfl(eg[j], L);
                 if (e >= 0)
                     digest = smc_mix(digest, ((uint64_t)step << 40) | ((uint64_t)((L - seg_base) * C + j) << 20) |
                                                  (uint64_t)e);
@@ -620,11 +625,12 @@ __device__ __forceinline__ void fire_receivers_serial(uint64_t (&mk)[C], int (&l
         for (int j = 0; j < C; ++j) {
             if ((__ballot(mk[0] != 0) >> L) & 1ull) {
                 const int c = (L - seg_base) * C + j;
-                int e_fired = -1;
+                int e_fired = -1;  // the network's index of the fired edge (digests)
                 if (lane == L) {
                     const int first = (int)m.locs[loc[0]].edge_first;
-                    e_fired = first + pick_receiver(mk[0], first, seed, run, step, (uint32_t)c, vars, m);
-                    loc[0] = fire_edge(e_fired, vars, m, q, goal, dummy_chan);
+                    const int i = pick_receiver(mk[0], first, seed, run, step, (uint32_t)c, vars, m);
+                    if (trace) e_fired = m.loc_edge0[loc[0]] + i;
+                    loc[0] = fire_edge(first + i, loc[0], vars, m, q, goal, dummy_chan);
                 }
                 wave_sync();
                 ++trans;

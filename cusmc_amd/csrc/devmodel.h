@@ -94,7 +94,7 @@ struct DevInv {             // x <= bound  (strict: x < bound), 16 bytes
 };
 
 struct DevEdge {            // 32 bytes
-    int32_t dst;
+    int32_t dst;            // destination relative to the source location
     int32_t channel;
     uint32_t weight;
     uint32_t guard_first;
@@ -176,6 +176,8 @@ struct DevModelT {
     const SMC_GLOBAL int32_t *comp_init_loc;
     const SMC_GLOBAL uint8_t *var_clock;   // [n_vars] 1: the variable is a clock
     const SMC_GLOBAL int32_t *comp_first_loc;  // [n_comp + 1] location range of each component
+    const SMC_GLOBAL int32_t *loc_edge0;       // [n_locs] the network's index of each location's first edge
+                                               // (edge records are shared: digests name network edges)
     int32_t n_comp;
     int32_t n_vars;
     int32_t n_clocks;

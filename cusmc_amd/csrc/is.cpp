@@ -599,7 +599,7 @@ int smc_is_create(smc_model *m, const smc_query *q, const smc_is_config *cfg_in,
                     sig.insert(sig.end(), {(int64_t)Lc.klass, (int64_t)Lc.type, (int64_t)Lc.n_inv, rate_key,
                                            (int64_t)Lc.edge_n});
                     for (int e = (int)Lc.edge_first; e < (int)Lc.edge_first + Lc.edge_n; ++e) {
-                        sig.push_back((int64_t)img.edges[e].dst - l0);
+                        sig.push_back((int64_t)l + img.edges[e].dst - l0);  // destinations are relative
                         sig.push_back((int64_t)img.edges[e].channel);
                     }
                 }

@@ -376,8 +376,9 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                         if (mkw) {
                             const int first = (int)m.locs[lj].edge_first;
                             const int i = pick_edge(mkw, first, p.seed, run, step, SMC_P_WINNER, (uint32_t)w, vars, m);
-                            e_fired = first + i;
-                            slot_set<C>(loc, wj, fire_edge(e_fired, vars, m, q, goal_local, new_chan));
+                            // the network's edge index (digests name it; records are shared)
+                            e_fired = TRACE ? m.loc_edge0[lj] + i : first + i;
+                            slot_set<C>(loc, wj, fire_edge(first + i, lj, vars, m, q, goal_local, new_chan));
                         }
                     }
                     wave_sync();
