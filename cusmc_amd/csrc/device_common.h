@@ -579,8 +579,12 @@ __device__ __forceinline__ bool fire_receivers_fast(const uint64_t (&mk)[C], int
         }
     }
     wave_sync();
-    for (int k = 1; k < S; k <<= 1) fired += __shfl_xor(fired, k);
-    trans += (uint32_t)fired;
+    // the segment's total, bit by bit (fired <= C <= 16): independent ballots
+    // instead of a chain of shuffles
+    uint32_t total = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) total += (uint32_t)__popcll(__ballot((fired >> b) & 1) & seg_mask) << b;
+    trans += total;
     if (trace) {
         for (uint64_t lanes_with = __ballot(any_fired) & seg_mask; lanes_with; lanes_with &= lanes_with - 1) {
             const int L = lowest_lane(lanes_with);

@@ -337,9 +337,11 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
                             wexp = (Lw - seg_base) * C + __shfl(node - C, Lw);
                         }
                         double gu = umin;
+                        if (__ballot(unif_j >= 0) & seg_mask) {  // some slot of the run drew a delay
 #pragma unroll
-                        for (int k = 0; k < 6; ++k)
-                            if ((1 << k) < S) gu = fmin(gu, seg_xchg(gu, k));
+                            for (int k = 0; k < 6; ++k)
+                                if ((1 << k) < S) gu = fmin(gu, seg_xchg(gu, k));
+                        }
                         int wu = 0x7FFFFFFF;
                         if (gu < INF) {
                             const int Lu = lowest_lane(__ballot(unif_j >= 0 && umin == gu) & seg_mask);
