@@ -224,6 +224,12 @@ ModelImage compile_model(const smc_network &n) {
         size_t ninv = img.invs.size() - d.inv_first;
         if (ninv > 65535) throw std::runtime_error("limit: more than 65535 invariants on a location");
         d.n_inv = (uint16_t)ninv;
+        if (ninv == 1 && img.invs[d.inv_first].bound_const) {
+            d.flags |= SMC_LF_INL_INV;
+            d.inl_value = img.invs[d.inv_first].bound_value;
+            d.inl_var = img.invs[d.inv_first].var;
+            d.inl_op = img.invs[d.inv_first].strict;
+        }
         d.klass = L.type > SMC_LOC_GOAL ? SMC_K_ZERO : (ninv > 0 ? SMC_K_BOUNDED : SMC_K_EXP);
         if (L.edge_count > SMC_MAX_EDGES_PER_LOC)
             throw std::runtime_error("limit: a location has more than 64 outgoing edges");
@@ -388,8 +394,8 @@ ModelImage compile_model(const smc_network &n) {
             if (never) continue;
             kind = clock_free ? 2 : 0;
         }
-        img.locs[l].flags = (uint8_t)((tf ? SMC_LF_TIME_FREE : 0) | (kind == 1 ? SMC_LF_DEAD : 0) |
-                                      (kind == 2 ? SMC_LF_GUARDED : 0));
+        img.locs[l].flags |= (uint8_t)((tf ? SMC_LF_TIME_FREE : 0) | (kind == 1 ? SMC_LF_DEAD : 0) |
+                                       (kind == 2 ? SMC_LF_GUARDED : 0));
         if (kind == 2 && img.locs[l].klass != SMC_K_ZERO) img.guarded = true;
         if (kind == 2) {
             // the blocking test: clock-free conjuncts, edge after edge
@@ -408,6 +414,14 @@ ModelImage compile_model(const smc_network &n) {
                 img.guards.back().flags |= SMC_GF_END_EDGE;
             }
             img.guards.back().flags |= SMC_GF_END_LOC;
+            const DevGuard &G = img.guards.back();
+            if (img.locs[l].block_first + 1 == img.guards.size() && SMC_REF_KIND(G.rhs) == SMC_REF_CONST &&
+                !(img.locs[l].flags & SMC_LF_INL_INV)) {
+                img.locs[l].flags |= SMC_LF_INL_BLK;
+                img.locs[l].inl_value = img.consts[G.rhs & 0x3FFFFFFFu];
+                img.locs[l].inl_var = G.var;
+                img.locs[l].inl_op = G.op;
+            }
         }
     }
     img.freeze = !clock_rate;

@@ -131,6 +131,28 @@ __device__ __forceinline__ T load_rec(const SMC_LDS T *p) {
 }
 #endif
 
+// A location record.  From the LDS-staged tables: all 48 bytes.  From global
+// memory: the first 32 (one pair of 16-byte loads, as many cache lines as
+// before the inline part existed), with the inline flags cleared, so the
+// general invariant and guard records are used.
+__device__ __forceinline__ DevLoc load_loc(const SMC_GLOBAL DevLoc *p) {
+    const SMC_GLOBAL uint4 *q = (const SMC_GLOBAL uint4 *)p;
+    uint4 v[3];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        v[i] = q[i];
+        asm("" : "+v"(v[i].x), "+v"(v[i].y), "+v"(v[i].z), "+v"(v[i].w));
+    }
+    v[2] = uint4{0u, 0u, 0u, 0u};
+    DevLoc r;
+    __builtin_memcpy(&r, v, sizeof(DevLoc));
+    r.flags &= (uint8_t)~(SMC_LF_INL_INV | SMC_LF_INL_BLK);
+    return r;
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ DevLoc load_loc(const SMC_LDS DevLoc *p) { return load_rec(p); }
+#endif
+
 // Table pointer of the global arena rebased onto its LDS copy.
 template <class T>
 __device__ __forceinline__ const SMC_LDS T *in_lds(const SMC_GLOBAL T *p, const SMC_GLOBAL uint8_t *base,
@@ -299,7 +321,7 @@ __device__ __forceinline__ bool guard_holds(int op, double x, double r) {
 // (check_enabled_edges simulation.cu:268-305, SyncInSerial :363-377).
 template <class M>
 __device__ __forceinline__ uint64_t enabled_mask(int loc, int recv_chan, const double *vars, const M &m) {
-    const DevLoc L = load_rec(m.locs + loc);
+    const DevLoc L = load_loc(m.locs + loc);
     const int ne = L.edge_n;
     uint64_t mask = 0;
     for (int i = 0; i < ne; ++i) {
@@ -379,7 +401,7 @@ __device__ __forceinline__ double dead_total(uint32_t deadmask, const int (&loc)
     for (int j = 0; j < C; ++j) {
         dl[j] = 0.0;
         if ((deadmask >> j) & 1u) {
-            const DevLoc L = load_rec(m.locs + loc[j]);
+            const DevLoc L = load_loc(m.locs + loc[j]);
             const double r = location_rate(L, vars, m, memo);
             dl[j] = r > 0.0 ? r : 0.0;
         }
@@ -424,6 +446,7 @@ __device__ __forceinline__ int fire_edge(int e, int src, double *vars, const M &
 // false clock-free conjunct (one pass over its blocking records).
 template <class M>
 __device__ __forceinline__ bool location_blocked(const DevLoc &L, const double *vars, const M &m) {
+    if (L.flags & SMC_LF_INL_BLK) return !guard_holds(L.inl_op, vars[L.inl_var], L.inl_value);
     bool e_blk = false;
     for (uint32_t k = L.block_first;; ++k) {
         const DevGuard G = load_rec(m.guards + k);
@@ -444,7 +467,7 @@ __device__ __forceinline__ uint32_t blocked_slots(const int (&loc)[C], const dou
     for (int j = 0; j < C; ++j) {
         const int c = sl * C + j;
         if (c >= n) continue;
-        const DevLoc L = load_rec(m.locs + loc[j]);
+        const DevLoc L = load_loc(m.locs + loc[j]);
         if ((L.flags & SMC_LF_GUARDED) && L.klass != SMC_K_ZERO && location_blocked(L, vars, m)) b |= 1u << j;
     }
     return b;
@@ -466,7 +489,7 @@ __device__ __forceinline__ bool absorbed(const int (&loc)[C], const double *vars
     for (int j = 0; j < C; ++j) {
         const int c = sl * C + j;
         if (can || c >= n) continue;
-        const DevLoc L = load_rec(m.locs + loc[j]);
+        const DevLoc L = load_loc(m.locs + loc[j]);
         if (L.klass == SMC_K_ZERO) { can = true; continue; }
         if ((L.flags & SMC_LF_DEAD) || ((blk >> j) & 1u)) continue;
         if (!(L.flags & SMC_LF_TIME_FREE)) { can = true; continue; }

@@ -62,7 +62,9 @@ enum {
                             // there is none): always blocked
     SMC_LF_GUARDED = 4,     // blocked or not by the variables: every spontaneous edge without
                             // `!=` has a conjunct that reads no clock (DevLoc.block_first)
-    SMC_LF_RECV_FAST = 8    // every receive edge commutes (see DevUpd)
+    SMC_LF_RECV_FAST = 8,   // every receive edge commutes (see DevUpd)
+    SMC_LF_INL_INV = 16,    // one invariant with a constant bound, inline in DevLoc.inl_*
+    SMC_LF_INL_BLK = 32     // the blocking test is one conjunct `var op constant`, inline
 };
 
 struct DevInsn {
@@ -71,7 +73,7 @@ struct DevInsn {
     uint16_t a, b, c;
 };
 
-struct DevLoc {             // 32 bytes, one pair of 16-byte loads
+struct DevLoc {             // 48 bytes: 32 of location, 16 of inline data
     uint8_t klass;
     uint8_t type;
     uint16_t n_inv;         // clock upper-bound invariants
@@ -83,6 +85,16 @@ struct DevLoc {             // 32 bytes, one pair of 16-byte loads
     uint8_t rate_const;
     uint8_t flags;          // SMC_LF_*
     uint32_t block_first;   // SMC_LF_GUARDED: the blocking test, guard records from here on
+    // The commonest invariant (one, constant bound) or blocking test (one
+    // conjunct `var op constant`) as plain values, read with the location
+    // from the LDS-staged tables (one dependent trip less in the race's slot
+    // loop).  Kernels reading the tables from global memory load the first
+    // 32 bytes only and take the general records (load_loc).
+    double inl_value;       // SMC_LF_INL_INV: the bound; SMC_LF_INL_BLK: the constant
+    uint16_t inl_var;       // the clock bounded, or the variable compared
+    uint8_t inl_op;         // SMC_LF_INL_INV: 1 if strict; SMC_LF_INL_BLK: smc_cons_op
+    uint8_t inl_pad0;
+    uint32_t inl_pad1;
 };
 
 struct DevInv {             // x <= bound  (strict: x < bound), 16 bytes

@@ -246,7 +246,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     rref[j] = 0xFFFFFFFFu;
                     const int c = sl * C + j;
                     if (c < n) {
-                        const DevLoc L = load_rec(m.locs + loc[j]);
+                        const DevLoc L = load_loc(m.locs + loc[j]);
                         if (reblk && (L.flags & SMC_LF_GUARDED) && L.klass != SMC_K_ZERO) {
                             const uint32_t bit = 1u << j;
                             blk = location_blocked(L, vars, m) ? (blk | bit) : (blk & ~bit);
@@ -256,7 +256,11 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                         } else {
                             bool bounded = false;
                             double M = DBL_MAX;
-                            for (int i = 0; i < L.n_inv; ++i) {
+                            if (L.flags & SMC_LF_INL_INV) {
+                                double ttb = L.inl_value - (vars[-1] - vars[L.inl_var]);
+                                if (L.inl_op) ttb -= 1e-6;
+                                if (ttb >= 0.0) { M = ttb; bounded = true; }
+                            } else for (int i = 0; i < L.n_inv; ++i) {
                                 const DevInv iv = load_rec(m.invs + L.inv_first + i);
                                 const double bound = iv.bound_const ? iv.bound_value : eval_ref(iv.bound, vars, m);
                                 double ttb = bound - (vars[-1] - vars[iv.var]);
@@ -299,7 +303,7 @@ __device__ __forceinline__ void is_advance_rows(const IsParams &P, const MT &m, 
                     for (uint32_t b = deadmask; b; b &= b - 1) {
                         const int j = __ffs(b) - 1;
                         const int lj = slot_get<C>(loc, j);
-                        const DevLoc L = load_rec(m.locs + lj);
+                        const DevLoc L = load_loc(m.locs + lj);
                         const double r = location_rate(L, vars, m, memo);
                         const double lf = r > 0.0 ? r : 0.0;
                         double lt = lf;
