@@ -52,9 +52,9 @@ SEED = 20261202
 # what they read for a 1 GiB device copy).  valu: SQ_INSTS_VALU (wave64 VALU
 # instructions) per race step.
 PROFILED = {
-    "fischer32": {"shape": (4, 3, 1), "traffic_per_run": 23.3, "valu_per_step": 118.5,
+    "fischer32": {"shape": (4, 3, 1), "traffic_per_run": 23.3, "valu_per_step": 113.9,
                   "source": "profiles/sim_kernel_fischer32.md"},
-    "sir350": {"shape": (64, 3, 0), "traffic_per_run": 55.9, "valu_per_step": 1639.2,
+    "sir350": {"shape": (64, 3, 0), "traffic_per_run": 65.7, "valu_per_step": 1565.7,
                "source": "profiles/sim_kernel_sir350.md"},
 }
 
