@@ -236,9 +236,9 @@ def bench_is(args) -> dict:
     # runs plain needs for +-10% at the exact p, and the time at its rate
     need = (1.959963984540054 / 0.1) ** 2 * (1 - exact) / exact
     plain_to_ci_s = plain_s if rel <= 0.1 else (need / plain_rate if n else math.nan)
-    # learned IS: create, train (5 rounds x 1e5), estimate to +-10%
+    # learned IS: create, train (--is-rounds x --is-runs), estimate to +-10%
     t1 = time.perf_counter()
-    s = cs.ImportanceSampler(sim, q, rounds=5, runs_per_round=100_000, epochs=args.epochs,
+    s = cs.ImportanceSampler(sim, q, rounds=args.is_rounds, runs_per_round=args.is_runs, epochs=args.epochs,
                              learning_rate=args.lr, seed=SEED)
     rounds = s.train()
     t_train = time.perf_counter() - t1
@@ -254,8 +254,8 @@ def bench_is(args) -> dict:
         "ms_per_step": is_s * 1e3, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64 (simulation), bf16/fp32 (MLP)",
         "data": "synthetic (SIR N=100 network generated from the seed)",
-        "config": {"workload": "is sir-100 Pr[<=50](<> I>=90)", "exact": exact, "rounds": 5,
-                   "runs_per_round": 100_000, "epochs": args.epochs, "learning_rate": args.lr},
+        "config": {"workload": "is sir-100 Pr[<=50](<> I>=90)", "exact": exact, "rounds": args.is_rounds,
+                   "runs_per_round": args.is_runs, "epochs": args.epochs, "learning_rate": args.lr},
         "is_seconds": is_s, "is_train_seconds": t_train, "is_estimate": est.estimate,
         "is_rel_half_width": est.rel_half_width, "is_runs": est.n_runs, "is_hits": est.hits, "is_ess": est.ess,
         "plain_seconds_to_ci": plain_to_ci_s, "plain_measured_seconds": plain_s, "plain_runs": n,
@@ -351,6 +351,8 @@ def main() -> int:
     ap.add_argument("--plain-cap-s", type=float, default=120.0, help="is: cap on the plain run's wall time")
     ap.add_argument("--epochs", type=int, default=6, help="is: epochs per cross-entropy round")
     ap.add_argument("--lr", type=float, default=1e-2, help="is: Adam step")
+    ap.add_argument("--is-rounds", type=int, default=4, help="is: cross-entropy rounds")
+    ap.add_argument("--is-runs", type=int, default=50_000, help="is: runs per cross-entropy round")
     ap.add_argument("--lanes", type=int, default=0, help="fixed lanes per run (default: autotuned)")
     ap.add_argument("--occupancy", type=int, default=0, choices=[0, 3],
                     help="with --lanes: register policy (0 unrestricted, 3 three waves per SIMD)")
