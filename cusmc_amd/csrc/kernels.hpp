@@ -76,7 +76,7 @@ SimVariant sim_variant_c4_o3();
 SimVariant sim_variant_c8_o3();
 SimVariant sim_variant_c16_o3();
 
-// occ: 0 registers unrestricted, 2 at most 256 (two waves per SIMD); tab:
+// occ: 0 registers unrestricted, 3 at most 168 (three waves per SIMD); tab:
 // tables staged in LDS (lp.table_bytes > 0); waves: total, a multiple of
 // group_waves (waves per workgroup, <= SMC_MAX_GROUP_WAVES)
 hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int occ, int waves, int group_waves,
