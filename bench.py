@@ -103,7 +103,7 @@ def bench_plain(args) -> dict:
     steps = sum(r.steps for r in results)
     kms = sum(r.kernel_ms for r in results)
     achieved = COST_PER_STEP[args.workload] * steps / (kms / 1e3)
-    used = (results[-1].lanes_per_run, policy, int(bool(tables)))
+    used = (results[-1].lanes_per_run, policy, int(tables))
     prof = PROFILED.get(args.workload, {})
     same_shape = prof.get("shape") == used
     traffic = prof["traffic_per_run"] * runs if same_shape and prof.get("traffic_per_run") else None
@@ -124,6 +124,7 @@ def bench_plain(args) -> dict:
         "data": "synthetic (seeded network generated as UPPAAL XML)",
         "config": {"workload": name, "runs_per_step": runs, "lanes_per_run": results[-1].lanes_per_run,
                    "waves": results[-1].waves, "occupancy_policy": policy, "tables_in_lds": bool(tables),
+                   "table_mode": int(tables),
                    "launch": "fixed" if args.lanes else "autotuned"},
         "transitions_per_sec": trans / wall,
         "race_steps_per_sec": steps / wall,
@@ -356,8 +357,9 @@ def main() -> int:
     ap.add_argument("--lanes", type=int, default=0, help="fixed lanes per run (default: autotuned)")
     ap.add_argument("--occupancy", type=int, default=0, choices=[0, 3],
                     help="with --lanes: register policy (0 unrestricted, 3 three waves per SIMD)")
-    ap.add_argument("--tables", type=int, default=0, choices=[0, 1],
-                    help="with --lanes: model tables in global memory (0) or staged in LDS (1)")
+    ap.add_argument("--tables", type=int, default=0, choices=[0, 1, 2],
+                    help="with --lanes: model tables in global memory (0), staged in LDS (1), or staged "
+                         "but for the location records (2)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-is", action="store_true", help="fischer32: skip the learned-IS leg")
     args = ap.parse_args()

@@ -36,7 +36,7 @@ namespace {
 struct Plan {
     int S = 1, C = 1, R = 64, stride = 1;
     int occ = 0;  // run-kernel build: 0 registers unrestricted, 3 three waves per SIMD
-    bool tab = false;     // tables staged in LDS
+    int tab = 0;          // tables staged in LDS: 0 none, 1 all hot tables, 2 all but the locations
     int group_waves = 1;  // waves per workgroup
     uint32_t table_bytes = 0, wave_lds = 0;
     size_t lds = 0;
@@ -47,11 +47,12 @@ struct Plan {
 
 constexpr size_t kLdsPerCu = 160 * 1024;
 
-Plan plan_launch(int P, int max_vars, int lanes, int occ, int tables, uint32_t table_bytes, int waves_req,
+Plan plan_launch(int P, int max_vars, int lanes, int occ, int tables, uint32_t table_bytes, uint32_t small_bytes,
+                 int waves_req,
                  int waves_per_cu, uint64_t total_runs, uint64_t max_runs_per_inst, bool trace, int cu_count) {
     Plan pl;
     if (occ != 0 && occ != 3) throw std::runtime_error("occupancy must be 0 or 3");
-    if (tables != 0 && tables != 1) throw std::runtime_error("tables must be 0 or 1");
+    if (tables < 0 || tables > 2) throw std::runtime_error("tables must be 0, 1 or 2");
     pl.occ = occ;
     int S = lanes ? lanes : (P < 64 ? P : 64);
     if (S <= 0 || S > 64 || (S & (S - 1))) throw std::runtime_error("lanes_per_run must be a power of two <= 64");
@@ -68,12 +69,13 @@ Plan plan_launch(int P, int max_vars, int lanes, int occ, int tables, uint32_t t
     const size_t wave = ((size_t)pl.R * pl.stride * sizeof(double) + (size_t)2 * levels * 64 * sizeof(double) + 15) & ~(size_t)15;
     if (wave > 60000) throw std::runtime_error("limit: variables of the packed runs exceed the LDS budget");
     pl.wave_lds = (uint32_t)wave;
-    pl.tab = tables == 1 && !trace;  // digest runs (parity checks) read the tables from global memory
+    pl.tab = trace ? 0 : tables;  // digest runs (parity checks) read the tables from global memory
     if (pl.tab) {
-        if (table_bytes == 0 || table_bytes % 16) throw std::runtime_error("internal: table size");
-        pl.table_bytes = table_bytes;
+        const uint32_t bytes = pl.tab == 1 ? table_bytes : small_bytes;
+        if (bytes == 0 || bytes % 16) throw std::runtime_error("internal: table size");
+        pl.table_bytes = bytes;
         pl.group_waves = SMC_MAX_GROUP_WAVES;
-        if (table_bytes + (size_t)pl.group_waves * wave > kLdsPerCu)
+        if (bytes + (size_t)pl.group_waves * wave > kLdsPerCu)
             throw std::runtime_error("limit: the model tables do not fit in LDS");
     }
     pl.lds = pl.table_bytes + (size_t)pl.group_waves * wave;
@@ -259,6 +261,8 @@ int smc_model_create(const smc_network *net, int device, smc_model **out) {
         hip_check(hipMemcpy(m->arena, host.data(), total, hipMemcpyHostToDevice), "upload model");
         auto at = [&](int i) { return (const void *)((const uint8_t *)m->arena + parts[i].off); };
         m->table_bytes = (uint32_t)parts[7].off;  // locs .. code: what the run kernel reads every step
+        m->small_off = (uint32_t)parts[1].off;    // invs .. code
+        m->small_bytes = (uint32_t)(parts[7].off - parts[1].off);
         DevModel &d = m->dm;
         d.locs = (decltype(d.locs))at(0);
         d.invs = (decltype(d.invs))at(1);
@@ -310,7 +314,7 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
                 ? launch : (m->has_tuned ? &m->tuned : nullptr);
         hip_check(hipSetDevice(m->device), "hipSetDevice");
         Plan pl = plan_launch(m->img.n_leaves, m->img.n_vars, use ? use->lanes_per_run : 0, use ? use->occupancy : 0,
-                              use ? use->tables : 0, m->table_bytes, use ? use->waves : 0,
+                              use ? use->tables : 0, m->table_bytes, m->small_bytes, use ? use->waves : 0,
                               use ? use->waves_per_cu : 0, n_runs, n_runs, trace, m->cu_count);
         m->runs.ensure(n_runs);
         pl.total_chunks = (n_runs + pl.chunk_runs - 1) / pl.chunk_runs;
@@ -330,7 +334,10 @@ int smc_run(smc_model *m, const smc_query *q, uint64_t seed, uint64_t run_begin,
         lp.total_chunks = pl.total_chunks;
         lp.counter = m->d_counter;
         lp.round = nullptr;
-        lp.tables = pl.tab ? (decltype(lp.tables))m->arena : nullptr;
+        lp.tables = pl.tab == 1   ? (decltype(lp.tables))m->arena
+                    : pl.tab == 2 ? (decltype(lp.tables))((const uint8_t *)m->arena + m->small_off)
+                                  : nullptr;
+        lp.table_mode = pl.tab;
         lp.table_bytes = pl.table_bytes;
         lp.wave_lds = pl.wave_lds;
         const uint32_t blocks = reduce_blocks(n_runs);
@@ -386,7 +393,7 @@ int smc_run_noalloc(smc_model *m, const smc_query *q, uint64_t seed, uint64_t ru
         hip_check(smc::noalloc_scratch(m->device, &scr), "scratch address");
         uint8_t *base = (uint8_t *)scr;
         Plan pl = plan_launch(m->img.n_leaves, m->img.n_vars, use ? use->lanes_per_run : 0, use ? use->occupancy : 0,
-                              use ? use->tables : 0, m->table_bytes, use ? use->waves : 0,
+                              use ? use->tables : 0, m->table_bytes, m->small_bytes, use ? use->waves : 0,
                               use ? use->waves_per_cu : 0, n_runs, n_runs, false, m->cu_count);
         pl.total_chunks = (n_runs + pl.chunk_runs - 1) / pl.chunk_runs;
         int waves = pl.waves;
@@ -419,7 +426,10 @@ int smc_run_noalloc(smc_model *m, const smc_query *q, uint64_t seed, uint64_t ru
         lp.total_chunks = pl.total_chunks;
         lp.counter = d_counter;
         lp.round = nullptr;
-        lp.tables = pl.tab ? (decltype(lp.tables))m->arena : nullptr;
+        lp.tables = pl.tab == 1   ? (decltype(lp.tables))m->arena
+                    : pl.tab == 2 ? (decltype(lp.tables))((const uint8_t *)m->arena + m->small_off)
+                                  : nullptr;
+        lp.table_mode = pl.tab;
         lp.table_bytes = pl.table_bytes;
         lp.wave_lds = pl.wave_lds;
 
@@ -545,11 +555,11 @@ int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sampl
         for (int S = P < 64 ? P : 64; S >= 1; S >>= 1) {
             if (P / S > 16) break;
             for (int occ : {0, 3})
-                for (int tab : {0, 1}) {
+                for (int tab : {0, 1, 2}) {
                     cands.push_back(smc_launch{S, 0, 0, occ, tab});
                     int blocks = 0;
                     try {
-                        Plan pl = plan_launch(P, m->img.n_vars, S, occ, tab, m->table_bytes, 0, 0, sample_runs,
+                        Plan pl = plan_launch(P, m->img.n_vars, S, occ, tab, m->table_bytes, m->small_bytes, 0, 0, sample_runs,
                                               sample_runs, false, m->cu_count);
                         hip_check(smc::sim_occupancy(pl.C, false, pl.occ, pl.tab, pl.group_waves, pl.lds, &blocks),
                                   "occupancy query");
@@ -566,7 +576,7 @@ int smc_autotune(smc_model *m, const smc_query *q, uint64_t seed, uint64_t sampl
         for (const smc_launch &c : cands) {
             uint64_t n = sample_runs;
             try {
-                Plan pl = plan_launch(P, m->img.n_vars, c.lanes_per_run, c.occupancy, c.tables, m->table_bytes, 0,
+                Plan pl = plan_launch(P, m->img.n_vars, c.lanes_per_run, c.occupancy, c.tables, m->table_bytes, m->small_bytes, 0,
                                       c.waves_per_cu, sample_runs, sample_runs, false, m->cu_count);
                 n = std::max<uint64_t>(n, 2ull * (uint64_t)pl.waves * (uint64_t)pl.R);
             } catch (const std::exception &) {
@@ -737,6 +747,7 @@ int smc_batch_create(const smc_batch_item *items, int n_items, int tune, smc_bat
             b->lp.counter = b->d_counter;
             b->lp.round = b->d_round;
             b->lp.tables = nullptr;
+            b->lp.table_mode = 0;
             b->lp.table_bytes = 0;
             b->lp.wave_lds = (uint32_t)sh.wave_lds;
             hip_check(hipMemcpy(b->d_all, sh.inst.data(), n_items * sizeof(InstDesc), hipMemcpyHostToDevice),

@@ -83,6 +83,8 @@ struct smc_model {
     DevModel dm{};
     void *arena = nullptr;
     uint32_t table_bytes = 0;  // arena prefix a run kernel may stage in LDS (the hot tables)
+    uint32_t small_off = 0;    // the hot tables after the location records: offset and bytes
+    uint32_t small_bytes = 0;
     int cu_count = 0;
     smc::capi::RunBuffers runs;
     InstDesc h_inst{};

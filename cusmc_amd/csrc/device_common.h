@@ -186,6 +186,36 @@ __device__ __forceinline__ void stage_tables(const SMC_GLOBAL uint8_t *src, uint
     __syncthreads();
 }
 
+// The model with every hot table but the location records read from the
+// staged copy, which then starts at the invariants (base: their global
+// address).
+__device__ __forceinline__ DevModelT<SmcMixedTables> lds_model_mixed(const DevModel &mg,
+                                                                     const SMC_GLOBAL uint8_t *base, double *lds) {
+    const SMC_LDS uint8_t *tab = (const SMC_LDS uint8_t *)(SMC_LDS double *)lds;
+    DevModelT<SmcMixedTables> m;
+    m.locs = mg.locs;
+    m.invs = in_lds(mg.invs, base, tab);
+    m.edges = in_lds(mg.edges, base, tab);
+    m.guards = in_lds(mg.guards, base, tab);
+    m.upds = in_lds(mg.upds, base, tab);
+    m.consts = in_lds(mg.consts, base, tab);
+    m.code = in_lds(mg.code, base, tab);
+    m.var_init = mg.var_init;
+    m.comp_init_loc = mg.comp_init_loc;
+    m.var_clock = mg.var_clock;
+    m.comp_first_loc = mg.comp_first_loc;
+    m.loc_edge0 = mg.loc_edge0;
+    m.n_comp = mg.n_comp;
+    m.n_vars = mg.n_vars;
+    m.n_clocks = mg.n_clocks;
+    m.n_leaves = mg.n_leaves;
+    m.n_locs = mg.n_locs;
+    m.n_edges = mg.n_edges;
+    m.freeze = mg.freeze;
+    m.guarded = mg.guarded;
+    return m;
+}
+
 // The model with its hot tables read from the staged copy.
 __device__ __forceinline__ DevModelT<SmcLdsTables> lds_model(const DevModel &mg, const SMC_GLOBAL uint8_t *base,
                                                              double *lds) {

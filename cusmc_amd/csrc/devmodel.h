@@ -27,6 +27,7 @@
 #pragma once
 
 #include <stdint.h>
+#include <type_traits>
 
 #define SMC_REF_CONST 0u
 #define SMC_REF_VAR 1u
@@ -172,6 +173,15 @@ struct SmcGlobalTables {
 };
 struct SmcLdsTables {
     template <class T> using ptr = const SMC_LDS T *;
+};
+// Location records from global memory, every other hot table from the LDS
+// copy: for networks whose many location records would crowd LDS while
+// their edge, guard, update and expression records (shared by template
+// instances) are few.
+struct SmcMixedTables {
+    template <class T>
+    using ptr = typename std::conditional<std::is_same<T, DevLoc>::value, const SMC_GLOBAL T *,
+                                          const SMC_LDS T *>::type;
 };
 
 // Everything the run kernel needs to know about one model; passed by value.

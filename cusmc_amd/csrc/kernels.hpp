@@ -43,7 +43,8 @@ struct LaunchParams {
     uint64_t total_chunks;
     unsigned long long *counter;  // chunks handed out (cleared by launch_prep)
     const uint64_t *round;      // optional: runs of round r start at run_begin + r * n_runs
-    const SMC_GLOBAL uint8_t *tables;  // tables staged in LDS (one instance): arena start
+    const SMC_GLOBAL uint8_t *tables;  // tables staged in LDS (one instance): first staged byte
+    int32_t table_mode;         // 0 none, 1 every hot table, 2 all but the location records
     uint32_t table_bytes;       // bytes staged (multiple of 16), 0 without staging
     uint32_t wave_lds;          // LDS bytes of one wave after the tables (multiple of 16)
 };
@@ -61,9 +62,9 @@ namespace smc {
 
 // One build of the run kernel (sim_variant.hip).
 struct SimVariant {
-    hipError_t (*launch)(const LaunchParams &lp, bool trace, bool tab, int waves, int group_waves, size_t lds,
+    hipError_t (*launch)(const LaunchParams &lp, bool trace, int tab, int waves, int group_waves, size_t lds,
                          hipStream_t s);
-    hipError_t (*occupancy)(bool trace, bool tab, int group_waves, size_t lds, int *blocks_per_cu);
+    hipError_t (*occupancy)(bool trace, int tab, int group_waves, size_t lds, int *blocks_per_cu);
 };
 SimVariant sim_variant_c1_o0();
 SimVariant sim_variant_c2_o0();
@@ -77,11 +78,11 @@ SimVariant sim_variant_c8_o3();
 SimVariant sim_variant_c16_o3();
 
 // occ: 0 registers unrestricted, 3 at most 168 (three waves per SIMD); tab:
-// tables staged in LDS (lp.table_bytes > 0); waves: total, a multiple of
+// table placement (lp.table_mode; 0 global memory); waves: total, a multiple of
 // group_waves (waves per workgroup, <= SMC_MAX_GROUP_WAVES)
 hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int occ, int waves, int group_waves,
                       size_t lds_bytes, hipStream_t stream);
-hipError_t sim_occupancy(int slots, bool trace, int occ, bool tab, int group_waves, size_t lds_bytes,
+hipError_t sim_occupancy(int slots, bool trace, int occ, int tab, int group_waves, size_t lds_bytes,
                          int *blocks_per_cu);
 hipError_t launch_reduce(const InstDesc *inst, int n_inst, uint32_t blocks_per_inst, ReducePartial *partials,
                          ReducePartial *acc, int accumulate, hipStream_t s);

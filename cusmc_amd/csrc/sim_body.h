@@ -64,7 +64,7 @@ namespace {
 // variables of the wave's 64/S runs: var_stride doubles each][race-tree levels: 2 log2(S)
 // doubles per lane, level-major]; lp.wave_lds covers the largest of these
 // over the launch's instances.
-template <int C, bool TRACE, bool TAB>
+template <int C, bool TRACE, int TAB>
 __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
     extern __shared__ __attribute__((aligned(16))) double lds[];
     __shared__ unsigned int chunk_next_of[SMC_MAX_GROUP_WAVES];
@@ -74,7 +74,9 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
     double *wbase = lds + (TAB ? lp.table_bytes / 8 : 0) + (size_t)wv * (lp.wave_lds / 8);
     const double INF = __builtin_huge_val();
     const uint64_t round = lp.round ? *lp.round : 0;
-    using Tables = typename std::conditional<TAB, SmcLdsTables, SmcGlobalTables>::type;
+    // TAB 1: every hot table staged in LDS; 2: all but the location records
+    using Tables = typename std::conditional<
+        TAB == 1, SmcLdsTables, typename std::conditional<TAB == 2, SmcMixedTables, SmcGlobalTables>::type>::type;
     if (TAB) stage_tables(lp.tables, lp.table_bytes, lds);
 
   for (;;) {
@@ -94,7 +96,8 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
     }
     const DevModel mg = lp.inst[lo].m;
     DevModelT<Tables> m;
-    if constexpr (TAB) m = lds_model(mg, lp.tables, lds);
+    if constexpr (TAB == 1) m = lds_model(mg, lp.tables, lds);
+    else if constexpr (TAB == 2) m = lds_model_mixed(mg, lp.tables, lds);
     else m = mg;
     const SimParams p = lp.inst[lo].p;
     // the instance's packing: S lanes per run, 64 / S runs per wave
@@ -465,12 +468,12 @@ __device__ __forceinline__ void sim_body(const LaunchParams &lp) {
 // for every slot count: 150-256 VGPRs), and at most 168 (three waves per
 // SIMD; up to four slots per lane fit, more spill).  Which is faster depends
 // on the packing; the autotuner measures both.
-template <int C, bool TRACE, bool TAB>
+template <int C, bool TRACE, int TAB>
 __global__ __launch_bounds__(64 * SMC_MAX_GROUP_WAVES) void sim_kernel(LaunchParams lp) {
     sim_body<C, TRACE, TAB>(lp);
 }
 
-template <int C, bool TRACE, bool TAB>
+template <int C, bool TRACE, int TAB>
 __global__ __launch_bounds__(64 * SMC_MAX_GROUP_WAVES) __attribute__((amdgpu_waves_per_eu(3))) void sim_kernel_o3(
     LaunchParams lp) {
     sim_body<C, TRACE, TAB>(lp);

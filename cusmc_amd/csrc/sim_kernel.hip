@@ -114,10 +114,10 @@ hipError_t launch_sim(const LaunchParams &lp, int slots, bool trace, int occ, in
                       size_t lds_bytes, hipStream_t stream) {
     const SimVariant v = variant(slots, occ);
     if (!v.launch || group_waves < 1 || group_waves > SMC_MAX_GROUP_WAVES || waves % group_waves) return hipErrorInvalidValue;
-    return v.launch(lp, trace, lp.table_bytes > 0, waves, group_waves, lds_bytes, stream);
+    return v.launch(lp, trace, lp.table_bytes > 0 ? lp.table_mode : 0, waves, group_waves, lds_bytes, stream);
 }
 
-hipError_t sim_occupancy(int slots, bool trace, int occ, bool tab, int group_waves, size_t lds_bytes,
+hipError_t sim_occupancy(int slots, bool trace, int occ, int tab, int group_waves, size_t lds_bytes,
                          int *blocks_per_cu) {
     const SimVariant v = variant(slots, occ);
     if (!v.occupancy) return hipErrorInvalidValue;

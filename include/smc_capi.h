@@ -80,8 +80,10 @@ typedef struct smc_launch {
                                   waves per SIMD                           */
     int32_t tables;            /* 0: model tables read from global memory;
                                   1: staged in LDS, one copy per workgroup
-                                  of four waves (smc_run only; batches read
-                                  them from global memory)                 */
+                                  of four waves; 2: staged in LDS but for the
+                                  location records, read from global memory
+                                  (smc_run only; batches read the tables
+                                  from global memory)                      */
 } smc_launch;
 
 /* Per-run outputs, all optional (NULL = not wanted), each n_runs long. */

@@ -130,15 +130,16 @@ def test_lds_tables_bit_exact_vs_oracle(name, models_dir):
     sim = cs.GpuSimulator(net)
     try:
         for S in lane_opts:
-            for occ in (0, 3):
-                print(f"[gpu parity, tables in LDS] {name} lanes_per_run={S} occupancy={occ} runs={runs}", flush=True)
-                res = sim.run(q, seed=seed, n_runs=runs, run_begin=3, lanes_per_run=S, occupancy=occ, tables=1,
+            for occ, tab in ((0, 1), (3, 1), (3, 2)):
+                print(f"[gpu parity, tables in LDS] {name} lanes_per_run={S} occupancy={occ} tables={tab} "
+                      f"runs={runs}", flush=True)
+                res = sim.run(q, seed=seed, n_runs=runs, run_begin=3, lanes_per_run=S, occupancy=occ, tables=tab,
                               per_run=keys)
                 for k in keys:
                     a, b = res.per_run[k], ro[k]
                     same = np.array_equal(a.view(np.uint64), b.view(np.uint64)) if k in ("value", "time") \
                         else np.array_equal(a, b)
-                    assert same, f"{name} S={S} occupancy={occ}: per-run {k} differs from the oracle"
+                    assert same, f"{name} S={S} occupancy={occ} tables={tab}: per-run {k} differs from the oracle"
                 assert res.hits == ref.hits and res.steps == ref.steps
     finally:
         sim.close()
