@@ -47,14 +47,14 @@ COST_PER_STEP = {"fischer32": 598.2, "sir350": 2214.6, "race8": 639.4}
 SEED = 20261202
 # Counter figures of the run kernel from rocprofv3 runs of this file (the
 # commands and raw values are in profiles/), for the launch shape (lanes per
-# run, register policy, tables in LDS) they were taken on; another shape
+# run, register policy, table placement) they were taken on; another shape
 # reports null.  traffic: HBM bytes per run (FETCH_SIZE + WRITE_SIZE, scaled by
 # what they read for a 1 GiB device copy).  valu: SQ_INSTS_VALU (wave64 VALU
 # instructions) per race step.
 PROFILED = {
-    "fischer32": {"shape": (4, 3, 1), "traffic_per_run": 23.3, "valu_per_step": 113.9,
+    "fischer32": {"shape": (4, 3, 1), "traffic_per_run": 23.1, "valu_per_step": 113.9,
                   "source": "profiles/sim_kernel_fischer32.md"},
-    "sir350": {"shape": (64, 3, 0), "traffic_per_run": 65.7, "valu_per_step": 1565.7,
+    "sir350": {"shape": (64, 3, 2), "traffic_per_run": 25.9, "valu_per_step": 1543.6,
                "source": "profiles/sim_kernel_sir350.md"},
 }
 
